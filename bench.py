@@ -1,0 +1,159 @@
+"""Benchmark of the range-analysis hot path on one MI355X.
+
+Workload (one "step"): the level-set tree of the ReLU MLP 3->256x6->1
+(seeded, median-shifted output bias) under affine_fixed range analysis, built
+on [-1,1]^3 to split depth 18 (64^3 finest cells), FP32.  ``value`` is boxes
+bounded per second over K timed steps (inputs resident: the model is uploaded
+before timing).  The same model's 1024x1024 ray cast (60 deg FOV, eye at
+distance 3, t in [0,6]) is timed beside it and reported as rays/s.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build targets one
+GPU per process and no multi-GPU figure is produced.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "neural-implicit-queries_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--depth", type=int, default=18)
+    ap.add_argument("--ray-res", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(params, target_s):
+    """The oracle (NumPy FP32 restatement of the reference) on host cores:
+    the first levels of the same tree, deepening until ~target_s of work."""
+    from oracle import funcs as ofuncs, tree as otree
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    f = ofuncs.OracleFunc(params, "affine_fixed", np.float32)
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    depth, boxes, secs = 6, 0, 0.0
+    while True:
+        t0 = time.perf_counter()
+        out = otree.construct_uniform_unknown_levelset_tree(f, lo, up, split_depth=depth)
+        dt = time.perf_counter() - t0
+        boxes, secs = out["stats"]["n_classified"], dt
+        if dt * 2.2 > target_s or depth >= 18:
+            break
+        depth += 1
+    return {"value": boxes / secs, "unit": "boxes/s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle tree of the same model to split depth {depth} "
+                      f"({boxes} boxes, {secs:.1f} s, NumPy FP32 with {threads} BLAS threads)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU per process; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from niq import models, implicit_mlp_utils, kd_tree, queries, render
+
+    name = "c2_relu_256x6"
+    params = models.config_model(name)
+    func, params = implicit_mlp_utils.implicit_from_params(params, "affine_fixed")
+    dims = [3] + [256] * 6 + [1]
+    flops_box = models.flops_per_box(dims, rows=5)
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+
+    def step():
+        return kd_tree.build_tree(func, params, lo, up, split_depth=args.depth)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_boxes = 0
+    for _ in range(args.steps):
+        _, st = step()
+        n_boxes += st["n_classified"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    boxes_per_s = n_boxes / elapsed
+
+    # dominant kernel: the fused classify pass, timed alone on torch's stream
+    # over the finest level's worth of boxes with HIP events
+    n_cls = 1 << 18
+    g = torch.Generator(device="cpu").manual_seed(0)
+    c = torch.rand((n_cls, 3), generator=g) * 2 - 1
+    h = torch.full((n_cls, 3), 1.0 / 64)
+    blo, bup = (c - h).cuda(), (c + h).cuda()
+    func.classify_box(params, blo, bup)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    ev0.record()
+    for _ in range(reps):
+        func.classify_box(params, blo, bup)
+    ev1.record()
+    torch.cuda.synchronize()
+    cls_ms = ev0.elapsed_time(ev1) / reps
+    achieved_tflops = flops_box * n_cls / (cls_ms * 1e-3) / 1e12
+
+    # rays (same model), one cast of res^2 rays
+    eye = np.array([0., 0., 3.], np.float32)
+    look, upd, _ = render.look_at(eye)
+    roots, dirs = render.generate_camera_rays(eye, look, upd, res=args.ray_res, fov_deg=60.)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    queries.cast_rays((func,), (params,), roots[:4096], dirs[:4096], opts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, hid, cnt, n_evals = queries.cast_rays((func,), (params,), roots, dirs, opts)
+    torch.cuda.synchronize()
+    ray_s = time.perf_counter() - t1
+    ray_steps = int(cnt.sum().item())
+
+    line = {
+        "metric": "boxes bounded/s (level-set tree, affine_fixed FP32)",
+        "value": boxes_per_s,
+        "unit": "boxes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ReLU MLP, He init, median-shifted output bias)",
+        "config": {"workload": f"levelset tree, ReLU MLP 3->256x6->1, affine_fixed, split depth {args.depth}",
+                   "boxes_per_step": n_boxes // args.steps, "parallelism": "single GPU"},
+        "rays": {"rays_per_s": args.ray_res ** 2 / ray_s, "ray_steps_per_s": ray_steps / ray_s,
+                 "res": args.ray_res, "seconds": ray_s, "hits": int((hid > 0).sum().item()),
+                 "n_evals": n_evals},
+        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_classify (affine_fixed, W=256)", "launch_ms": cls_ms,
+                     "flops_per_box": flops_box, "boxes_per_launch": n_cls},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(params, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * niq.h - C ABI of the MI355X (gfx950) range-analysis hot path.
+ *
+ * Plain pointers and sizes only.  Every device pointer is caller-owned HIP
+ * device memory; `stream` is a hipStream_t (NULL = default stream).  Calls
+ * are asynchronous on `stream` unless documented as synchronising.  A model
+ * handle is immutable after creation and may be shared by threads and
+ * streams.  Errors: a negative niq_status, with a message from
+ * niq_last_error() (thread-local).
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference repository).
+ */
+#ifndef NIQ_H
+#define NIQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int niq_status;
+#define NIQ_OK 0
+#define NIQ_ERR_INVALID (-1)      /* bad argument (cf. ValueError in src/mlp.py:131,142) */
+#define NIQ_ERR_UNSUPPORTED (-2)  /* shape/mode not supported by any kernel */
+#define NIQ_ERR_CAPACITY (-3)     /* an output or workspace buffer is too small */
+#define NIQ_ERR_HIP (-4)          /* HIP runtime error */
+
+/* activations (src/mlp.py:283-300; TANH has no reference rule) */
+#define NIQ_ACT_NONE 0
+#define NIQ_ACT_RELU 1
+#define NIQ_ACT_ELU 2
+#define NIQ_ACT_SIN 3
+#define NIQ_ACT_TANH 4
+
+/* range-analysis modes (src/affine.py:62-76, src/implicit_mlp_utils.py:24-61) */
+#define NIQ_MODE_INTERVAL 0
+#define NIQ_MODE_AFFINE_FIXED 1
+#define NIQ_MODE_AFFINE_TRUNCATE 2
+#define NIQ_MODE_AFFINE_APPEND 3
+#define NIQ_MODE_AFFINE_ALL 4
+#define NIQ_MODE_SLOPE_INTERVAL 5
+
+/* arithmetic of the error-term contraction */
+#define NIQ_PREC_FP32 0
+#define NIQ_PREC_BF16 1
+
+/* sign codes (src/implicit_function.py:11-13) */
+#define NIQ_SIGN_UNKNOWN 0
+#define NIQ_SIGN_POSITIVE 1
+#define NIQ_SIGN_NEGATIVE 2
+
+typedef struct niq_model niq_model;
+
+typedef struct niq_eval_cfg {
+  int mode;            /* NIQ_MODE_* */
+  int precision;       /* NIQ_PREC_* */
+  int truncate_count;  /* affine_truncate: rows kept */
+  int n_append;        /* affine_append: rows appended per activation */
+  float recheck_tol;   /* > 0: FP64 re-test of cells within tol*scale of the threshold */
+  int reserved[3];
+} niq_eval_cfg;
+
+/* ---- model ------------------------------------------------------------
+ * Replaces generate_implicit_from_file / mlp.load (src/implicit_mlp_utils.py:12-64,
+ * src/mlp.py:173-185): the npz op list, lowered by the caller to n_layers dense
+ * layers.  Layer i maps dims[i] -> dims[i+1] with W[i] row-major (dims[i],
+ * dims[i+1]) (the reference's A, src/mlp.py:218-249, 253-258) and bias b[i]
+ * (nullable); acts[i] is applied after layer i (ignored for the last layer, whose
+ * output width must be 1: squeeze_last, src/mlp.py:328-332).  Host pointers;
+ * weights are copied to the current device.  Synchronising. */
+niq_status niq_model_create(int n_layers, const int* dims, const int* acts,
+                            const float* const* W, const float* const* b,
+                            niq_model** out);
+void niq_model_destroy(niq_model* m);
+/* bytes of device memory the model holds */
+size_t niq_model_device_bytes(const niq_model* m);
+
+/* ---- per-box classification --------------------------------------------
+ * Replaces ImplicitFunction.classify_box (src/implicit_function.py:28-37) with
+ * AffineImplicitFunction / SlopeIntervalImplicitFunction.classify_general_box
+ * (src/affine.py:34-55, src/slope_interval.py:29-51), vmapped over n boxes.
+ * lower/upper: (n, d) float32.  codes: (n,) int32 NIQ_SIGN_*.
+ * bounds: optional (n, 2) float32 [may_lower, may_upper]. */
+niq_status niq_classify_boxes(const niq_model* m, const niq_eval_cfg* cfg,
+                              const float* lower, const float* upper, int64_t n, int d,
+                              float offset, int32_t* codes, float* bounds, void* stream);
+
+/* General boxes: center (n, d), vecs (n, v, d) (src/affine.py:109-117). */
+niq_status niq_classify_general_boxes(const niq_model* m, const niq_eval_cfg* cfg,
+                                      const float* center, const float* vecs, int64_t n,
+                                      int v, int d, float offset, int32_t* codes,
+                                      float* bounds, void* stream);
+
+/* ---- plain evaluation ----------------------------------------------------
+ * Replaces ImplicitFunction.__call__ (src/affine.py:26-28 -> src/mlp.py:96-113,
+ * default rules) vmapped over points: x (n, d) -> out (n,). */
+niq_status niq_eval_points(const niq_model* m, const float* x, int64_t n, int d,
+                           float* out, void* stream);
+
+/* ---- level-set tree --------------------------------------------------------
+ * Replaces construct_uniform_unknown_levelset_tree (src/kd_tree.py:103-218) and
+ * its level step (src/kd_tree.py:19-100) with bucketing (src/bucketing.py:16-32).
+ * All levels run on `stream` with device-side counts; the call synchronises once
+ * at the end (and once per level when node_terminate_thresh > 0). */
+typedef struct niq_tree_params {
+  float lower[3], upper[3];
+  int split_depth;                 /* < 0: none (then node_terminate_thresh > 0) */
+  int64_t node_terminate_thresh;   /* <= 0: none */
+  float offset;
+  int batch_process_size;          /* B of the reference; fixes the output order */
+  int with_interior_nodes;
+  int with_exterior_nodes;
+} niq_tree_params;
+
+typedef struct niq_tree_result {
+  /* caller-provided device buffers (n, 3) float32 and their capacities (rows) */
+  float* unknown_lower; float* unknown_upper; int64_t unknown_capacity;
+  float* interior_lower; float* interior_upper; int64_t interior_capacity;
+  float* exterior_lower; float* exterior_upper; int64_t exterior_capacity;
+  /* filled on return */
+  int64_t n_unknown, n_interior, n_exterior;
+  int64_t n_classified;            /* boxes bounded */
+  int64_t n_recheck;               /* cells re-tested in FP64 */
+  int n_levels;
+  int32_t level_n[64], level_interior[64], level_exterior[64];
+} niq_tree_result;
+
+/* Workspace (device bytes) a tree build needs: level ping-pong lists, codes,
+ * per-batch counters. */
+niq_status niq_tree_workspace_bytes(const niq_model* m, const niq_eval_cfg* cfg,
+                                    const niq_tree_params* p, size_t* bytes);
+niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg,
+                          const niq_tree_params* p, void* workspace, size_t workspace_bytes,
+                          niq_tree_result* res, void* stream);
+
+/* ---- ray casting ------------------------------------------------------------
+ * Replaces cast_rays / cast_rays_iter (src/queries.py:134-175, 39-132) with the
+ * options of get_default_cast_opts (src/queries.py:23-36). */
+typedef struct niq_cast_opts {
+  float hit_eps, max_dist;
+  int n_max_step, n_substeps;
+  float safety_factor, interval_grow_fac, interval_shrink_fac;
+  float init_step;   /* interval_init_size * max_dist, rounded once to float32 */
+} niq_cast_opts;
+
+/* models[n_models] (func ids 1..n_models); roots, dirs (n, 3) float32;
+ * out_t (n,) float32, out_hit_id (n,) int32, out_count (n,) int32;
+ * workspace >= niq_cast_workspace_bytes.  Synchronising; *n_evals receives
+ * the reference's N_evals (evaluations including its bucket padding). */
+size_t niq_cast_workspace_bytes(const niq_cast_opts* opts);
+niq_status niq_cast_rays(const niq_model* const* models, int n_models, const niq_eval_cfg* cfg,
+                         const float* roots, const float* dirs, int64_t n,
+                         const niq_cast_opts* opts, float* out_t, int32_t* out_hit_id,
+                         int32_t* out_count, void* workspace, size_t workspace_bytes,
+                         int64_t* n_evals, void* stream);
+
+/* ---- diagnostics -------------------------------------------------------- */
+const char* niq_last_error(void);
+int niq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NIQ_H */

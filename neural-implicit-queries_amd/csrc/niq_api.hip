@@ -1,0 +1,563 @@
+// C ABI of the gfx950 range-analysis hot path (declarations in include/niq.h).
+#include "niq.h"
+#include "niq_kernels.h"
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+niq_status fail(niq_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      return fail(NIQ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+inline int round16(int x) { return (x + 15) / 16 * 16; }
+
+// Template width that holds every padded layer width of a model.
+inline int template_width(int wmax) {
+  if (wmax <= 64) return 64;
+  if (wmax <= 128) return 128;
+  if (wmax <= 256) return 256;
+  if (wmax <= 512) return 512;
+  return -1;
+}
+
+int num_cus() {
+  static int cus = -1;
+  if (cus < 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+  }
+  return cus;
+}
+
+// Persistent grid: resident workgroups of kernel `f` over the whole device.
+template <class F>
+long long persistent_blocks(F f, int threads) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find((const void*)f);
+  int occ = 0;
+  if (it == cache.end()) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, threads, 0) != hipSuccess || occ < 1) occ = 1;
+    cache[(const void*)f] = occ;
+  } else {
+    occ = it->second;
+  }
+  return (long long)occ * num_cus();
+}
+
+inline unsigned grid_for(long long work_blocks, long long persist) {
+  long long g = std::min(work_blocks, persist);
+  return (unsigned)std::max(1LL, g);
+}
+
+}  // namespace
+
+struct niq_model {
+  int n_layers = 0;
+  int in_dim = 0;
+  int wt = 0;                  // template width
+  std::vector<int> dims, acts;
+  DevModel host;               // with device pointers
+  DevModel* dev = nullptr;     // device copy of `host`
+  void* weights = nullptr;     // packed weights and biases
+  size_t bytes = 0;
+};
+
+extern "C" {
+
+const char* niq_last_error(void) { return g_err.c_str(); }
+int niq_version(void) { return 1; }
+
+niq_status niq_model_create(int n_layers, const int* dims, const int* acts, const float* const* W,
+                            const float* const* b, niq_model** out) {
+  if (!out || !dims || !acts || !W) return fail(NIQ_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (n_layers < 1 || n_layers > NIQ_MAX_LAYERS)
+    return fail(NIQ_ERR_INVALID, "n_layers must be in [1, " + std::to_string(NIQ_MAX_LAYERS) + "]");
+  if (dims[0] < 1 || dims[0] > 16) return fail(NIQ_ERR_INVALID, "input dimension must be in [1, 16]");
+  if (dims[n_layers] != 1) return fail(NIQ_ERR_INVALID, "the last layer must have one output (squeeze_last)");
+  int wmax = 16;
+  for (int i = 0; i <= n_layers; ++i) {
+    if (dims[i] < 1) return fail(NIQ_ERR_INVALID, "layer widths must be positive");
+    wmax = std::max(wmax, round16(dims[i]));
+  }
+  for (int i = 0; i < n_layers; ++i)
+    if (acts[i] < NIQ_ACT_NONE || acts[i] > NIQ_ACT_TANH) return fail(NIQ_ERR_INVALID, "unknown activation");
+  const int wt = template_width(wmax);
+  if (wt < 0) return fail(NIQ_ERR_UNSUPPORTED, "layer widths above 512 are not supported");
+
+  // Pack each layer's W (in, out) into MFMA A-fragments: fragment (ft, kb) is
+  // 64 lanes x 4 floats, lane l / slot j holding W[16 kb + 4 (l / 16) + j][16 ft + l % 16].
+  std::vector<size_t> woff(n_layers), boff(n_layers);
+  size_t total = 0;
+  for (int i = 0; i < n_layers; ++i) {
+    const size_t ip = round16(dims[i]), op = round16(dims[i + 1]);
+    woff[i] = total;
+    total += ip * op * sizeof(float);
+    boff[i] = total;
+    total += op * sizeof(float);
+    total = (total + 255) / 256 * 256;
+  }
+  std::vector<unsigned char> host(total, 0);
+  for (int i = 0; i < n_layers; ++i) {
+    const int in = dims[i], outd = dims[i + 1];
+    const int ip = round16(in), op = round16(outd);
+    const int nkb = ip / 16;
+    float* wp = reinterpret_cast<float*>(host.data() + woff[i]);
+    for (int ft = 0; ft < op / 16; ++ft)
+      for (int kb = 0; kb < nkb; ++kb)
+        for (int l = 0; l < 64; ++l)
+          for (int j = 0; j < 4; ++j) {
+            const int k = 16 * kb + 4 * (l / 16) + j;
+            const int f = 16 * ft + (l % 16);
+            const float v = (k < in && f < outd) ? W[i][(size_t)k * outd + f] : 0.f;
+            wp[((size_t)(ft * nkb + kb) * 64 + l) * 4 + j] = v;
+          }
+    float* bp = reinterpret_cast<float*>(host.data() + boff[i]);
+    for (int f = 0; f < outd; ++f) bp[f] = (b && b[i]) ? b[i][f] : 0.f;
+  }
+
+  niq_model* m = new niq_model();
+  m->n_layers = n_layers;
+  m->in_dim = dims[0];
+  m->wt = wt;
+  m->dims.assign(dims, dims + n_layers + 1);
+  m->acts.assign(acts, acts + n_layers);
+  hipError_t e = hipMalloc(&m->weights, total);
+  if (e == hipSuccess) e = hipMemcpy(m->weights, host.data(), total, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&m->dev), sizeof(DevModel));
+  if (e != hipSuccess) {
+    niq_model_destroy(m);
+    return fail(NIQ_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
+  }
+  std::memset(&m->host, 0, sizeof(DevModel));
+  m->host.n_layers = n_layers;
+  m->host.wmax = wmax;
+  m->host.in_dim = dims[0];
+  for (int i = 0; i < n_layers; ++i) {
+    DevLayer& L = m->host.L[i];
+    L.in_pad = round16(dims[i]);
+    L.out_pad = round16(dims[i + 1]);
+    L.act = (i + 1 < n_layers) ? acts[i] : ACT_NONE;
+    L.Wp = reinterpret_cast<const floatx4*>(static_cast<unsigned char*>(m->weights) + woff[i]);
+    L.bias = reinterpret_cast<const float*>(static_cast<unsigned char*>(m->weights) + boff[i]);
+  }
+  e = hipMemcpy(m->dev, &m->host, sizeof(DevModel), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    niq_model_destroy(m);
+    return fail(NIQ_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
+  }
+  m->bytes = total + sizeof(DevModel);
+  *out = m;
+  return NIQ_OK;
+}
+
+void niq_model_destroy(niq_model* m) {
+  if (!m) return;
+  if (m->weights) (void)hipFree(m->weights);
+  if (m->dev) (void)hipFree(m->dev);
+  delete m;
+}
+
+size_t niq_model_device_bytes(const niq_model* m) { return m ? m->bytes : 0; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------- dispatch
+namespace {
+
+// Launch k_classify for (KIND, V) at template width W.
+template <int KIND, int V, int W>
+niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
+  constexpr int R = Planes<KIND, V, 0>::R;
+  if constexpr (!Fits<R, NB, W>::value) {
+    return fail(NIQ_ERR_UNSUPPORTED, "range form does not fit in LDS at this width");
+  } else {
+    auto k = k_classify<KIND, V, NB, NW, W>;
+    const long long tiles = (max_n + NB - 1) / NB;
+    if (tiles == 0) return NIQ_OK;
+    const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+    hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, a);
+    HIPCHK(hipGetLastError());
+    return NIQ_OK;
+  }
+}
+
+template <int KIND, int V>
+niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  switch (m->wt) {
+    case 64: return launch_classify_w<KIND, V, 64>(m, a, max_n, s);
+    case 128: return launch_classify_w<KIND, V, 128>(m, a, max_n, s);
+    case 256: return launch_classify_w<KIND, V, 256>(m, a, max_n, s);
+    case 512: return launch_classify_w<KIND, V, 512>(m, a, max_n, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
+
+// Route a classification by (mode, number of box vectors).
+niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec,
+                             const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
+  const int prec = cfg ? cfg->precision : NIQ_PREC_FP32;
+  if (prec != NIQ_PREC_FP32) return fail(NIQ_ERR_UNSUPPORTED, "precision mode not available for this call");
+  if (mode == NIQ_MODE_INTERVAL) return launch_classify<0, 0>(m, a, max_n, s);
+  if (mode == NIQ_MODE_AFFINE_FIXED) {
+    switch (nvec) {
+      case 1: return launch_classify<0, 1>(m, a, max_n, s);
+      case 2: return launch_classify<0, 2>(m, a, max_n, s);
+      case 3: return launch_classify<0, 3>(m, a, max_n, s);
+    }
+    return fail(NIQ_ERR_UNSUPPORTED, "affine_fixed supports 1..3 box vectors");
+  }
+  if (mode == NIQ_MODE_SLOPE_INTERVAL) {
+    switch (nvec) {
+      case 1: return launch_classify<1, 1>(m, a, max_n, s);
+      case 2: return launch_classify<1, 2>(m, a, max_n, s);
+      case 3: return launch_classify<1, 3>(m, a, max_n, s);
+    }
+    return fail(NIQ_ERR_UNSUPPORTED, "slope_interval supports 1..3 box vectors");
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "mode not available in the fused kernel");
+}
+
+template <int W>
+niq_status launch_points_w(const niq_model* m, const float* x, int d, long long n, float* out, hipStream_t s) {
+  constexpr int NB = TileCfg<W>::NBP, NW = TileCfg<W>::NW;
+  auto k = k_points<NB, NW, W>;
+  const long long tiles = (n + NB - 1) / NB;
+  if (tiles == 0) return NIQ_OK;
+  const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+  hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, x, d, n, out);
+  HIPCHK(hipGetLastError());
+  return NIQ_OK;
+}
+
+niq_status check_model(const niq_model* m, int d) {
+  if (!m) return fail(NIQ_ERR_INVALID, "null model");
+  if (d != m->in_dim) return fail(NIQ_ERR_INVALID, "point dimension does not match the model input");
+  return NIQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+niq_status niq_classify_boxes(const niq_model* m, const niq_eval_cfg* cfg, const float* lower,
+                              const float* upper, int64_t n, int d, float offset, int32_t* codes,
+                              float* bounds, void* stream) {
+  if (niq_status st = check_model(m, d)) return st;
+  if (n < 0 || (n > 0 && (!lower || !upper || !codes))) return fail(NIQ_ERR_INVALID, "bad buffers");
+  ClassifyArgs a{};
+  a.lower = lower; a.upper = upper; a.d = d; a.n = n; a.offset = offset;
+  a.codes = codes; a.bounds = bounds;
+  return classify_dispatch(m, cfg, d, a, n, (hipStream_t)stream);
+}
+
+niq_status niq_classify_general_boxes(const niq_model* m, const niq_eval_cfg* cfg, const float* center,
+                                      const float* vecs, int64_t n, int v, int d, float offset,
+                                      int32_t* codes, float* bounds, void* stream) {
+  if (niq_status st = check_model(m, d)) return st;
+  if (v < 1) return fail(NIQ_ERR_INVALID, "need at least one box vector");
+  if (n < 0 || (n > 0 && (!center || !vecs || !codes))) return fail(NIQ_ERR_INVALID, "bad buffers");
+  ClassifyArgs a{};
+  a.center = center; a.vecs = vecs; a.vin = v; a.d = d; a.n = n; a.offset = offset;
+  a.codes = codes; a.bounds = bounds;
+  const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
+  return classify_dispatch(m, cfg, mode == NIQ_MODE_INTERVAL ? 0 : v, a, n, (hipStream_t)stream);
+}
+
+niq_status niq_eval_points(const niq_model* m, const float* x, int64_t n, int d, float* out, void* stream) {
+  if (niq_status st = check_model(m, d)) return st;
+  if (n < 0 || (n > 0 && (!x || !out))) return fail(NIQ_ERR_INVALID, "bad buffers");
+  hipStream_t s = (hipStream_t)stream;
+  switch (m->wt) {
+    case 64: return launch_points_w<64>(m, x, d, n, out, s);
+    case 128: return launch_points_w<128>(m, x, d, n, out, s);
+    case 256: return launch_points_w<256>(m, x, d, n, out, s);
+    case 512: return launch_points_w<512>(m, x, d, n, out, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
+
+// ----------------------------------------------------------- tree build
+namespace {
+
+struct TreeLayout {
+  long long cap;      // max nodes in any level list
+  long long nbmax;    // max batches per level
+  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], total;
+};
+
+niq_status tree_layout(const niq_tree_params* p, TreeLayout* L) {
+  if (!p) return fail(NIQ_ERR_INVALID, "null params");
+  const int B = p->batch_process_size;
+  if (B < 1 || (B & (B - 1)) || B > (1 << 24))
+    return fail(NIQ_ERR_INVALID, "batch_process_size must be a power of 2 (a factor of the bucket sizes)");
+  if (p->split_depth < 0 && p->node_terminate_thresh <= 0)
+    return fail(NIQ_ERR_INVALID, "must specify at least one of node_terminate_thresh or split_depth");
+  long long cap = LLONG_MAX;
+  if (p->split_depth >= 0) {
+    if (p->split_depth > 30) return fail(NIQ_ERR_UNSUPPORTED, "split_depth above 30");
+    cap = 1LL << p->split_depth;
+  }
+  if (p->node_terminate_thresh > 0) cap = std::min(cap, 2LL * p->node_terminate_thresh);
+  cap = std::max(cap, 1LL);
+  if (cap > (1LL << 30)) return fail(NIQ_ERR_UNSUPPORTED, "tree level capacity above 2^30 nodes");
+  L->cap = cap;
+  L->nbmax = (cap + B - 1) / B;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 255) / 256 * 256; return r; };
+  L->off_counters = take(sizeof(TreeCounters));
+  L->off_codes = take(sizeof(int) * cap);
+  L->off_bcnt = take(sizeof(int) * 4 * L->nbmax);
+  L->off_boff = take(sizeof(int) * 4 * L->nbmax);
+  L->off_lvl[0] = take(sizeof(float) * 6 * cap);
+  L->off_lvl[1] = take(sizeof(float) * 6 * cap);
+  L->total = o;
+  return NIQ_OK;
+}
+
+__global__ void k_tree_init(TreeCounters* C, float* lo, float* up, float l0, float l1, float l2, float u0,
+                            float u1, float u2) {
+  if (threadIdx.x == 0) {
+    C->n[0] = 1; C->n[1] = 0; C->n_int = 0; C->n_ext = 0; C->n_classified = 0; C->overflow = 0;
+    lo[0] = l0; lo[1] = l1; lo[2] = l2;
+    up[0] = u0; up[1] = u1; up[2] = u2;
+  }
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) { C->level_n[i] = 0; C->level_int[i] = 0; C->level_ext[i] = 0; }
+}
+
+}  // namespace
+
+niq_status niq_tree_workspace_bytes(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p,
+                                    size_t* bytes) {
+  (void)m; (void)cfg;
+  TreeLayout L;
+  if (niq_status st = tree_layout(p, &L)) return st;
+  if (bytes) *bytes = L.total;
+  return NIQ_OK;
+}
+
+niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
+                          size_t ws_bytes, niq_tree_result* res, void* stream) {
+  if (niq_status st = check_model(m, 3)) return st;
+  if (!res) return fail(NIQ_ERR_INVALID, "null result");
+  TreeLayout L;
+  if (niq_status st = tree_layout(p, &L)) return st;
+  if (!ws || ws_bytes < L.total)
+    return fail(NIQ_ERR_CAPACITY, "tree workspace too small: need " + std::to_string(L.total) + " bytes");
+  if (!res->unknown_lower || !res->unknown_upper || res->unknown_capacity < 0)
+    return fail(NIQ_ERR_INVALID, "unknown-node output buffers are required");
+  if (p->with_interior_nodes && (!res->interior_lower || !res->interior_upper))
+    return fail(NIQ_ERR_INVALID, "interior output buffers are required");
+  if (p->with_exterior_nodes && (!res->exterior_lower || !res->exterior_upper))
+    return fail(NIQ_ERR_INVALID, "exterior output buffers are required");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* base = static_cast<unsigned char*>(ws);
+  TreeCounters* C = reinterpret_cast<TreeCounters*>(base + L.off_counters);
+  int* codes = reinterpret_cast<int*>(base + L.off_codes);
+  int* bcnt = reinterpret_cast<int*>(base + L.off_bcnt);
+  int* boff = reinterpret_cast<int*>(base + L.off_boff);
+  float* lvl[2] = {reinterpret_cast<float*>(base + L.off_lvl[0]), reinterpret_cast<float*>(base + L.off_lvl[1])};
+  const long long cap = L.cap;
+  const int B = p->batch_process_size;
+
+  hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, C, lvl[0], lvl[0] + 3 * cap, p->lower[0], p->lower[1],
+                     p->lower[2], p->upper[0], p->upper[1], p->upper[2]);
+  HIPCHK(hipGetLastError());
+
+  const int n_levels = p->split_depth >= 0 ? p->split_depth + 1 : 64;
+  int level = 0;
+  for (; level < std::min(n_levels, 62); ++level) {
+    bool quit_next = (level + 1 == n_levels);
+    if (p->node_terminate_thresh > 0) {
+      int n_host = 0;
+      HIPCHK(hipMemcpyAsync(&n_host, &C->n[level & 1], sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      quit_next = quit_next || (n_host >= p->node_terminate_thresh);
+    }
+    const int split = quit_next ? 0 : 1;
+    const long long max_n = level < 40 ? std::min(cap, 1LL << level) : cap;
+    float* cur_lo = lvl[level & 1];
+    float* cur_up = cur_lo + 3 * cap;
+
+    ClassifyArgs a{};
+    a.lower = cur_lo; a.upper = cur_up; a.d = 3; a.n_dev = &C->n[level & 1]; a.offset = p->offset;
+    a.codes = codes;
+    if (niq_status st = classify_dispatch(m, cfg, 3, a, max_n, s)) return st;
+
+    const unsigned nb = (unsigned)((max_n + B - 1) / B);
+    hipLaunchKernelGGL(k_tree_count, dim3(nb), dim3(256), 0, s, codes, C, level, B, bcnt);
+    HIPCHK(hipGetLastError());
+
+    TreeScatterArgs sa{};
+    sa.codes = codes; sa.lower = cur_lo; sa.upper = cur_up;
+    if (split) {
+      sa.next_lower = lvl[(level + 1) & 1];
+      sa.next_upper = sa.next_lower + 3 * cap;
+      sa.cap_next = cap;
+    } else {
+      sa.next_lower = res->unknown_lower;
+      sa.next_upper = res->unknown_upper;
+      sa.cap_next = res->unknown_capacity;
+    }
+    if (p->with_interior_nodes) { sa.int_lower = res->interior_lower; sa.int_upper = res->interior_upper; sa.cap_int = res->interior_capacity; }
+    if (p->with_exterior_nodes) { sa.ext_lower = res->exterior_lower; sa.ext_upper = res->exterior_upper; sa.cap_ext = res->exterior_capacity; }
+    const long long cap_int = p->with_interior_nodes ? res->interior_capacity : LLONG_MAX;
+    const long long cap_ext = p->with_exterior_nodes ? res->exterior_capacity : LLONG_MAX;
+    hipLaunchKernelGGL(k_tree_scan, dim3(1), dim3(1024), 0, s, bcnt, boff, C, level, B, split, sa.cap_next, cap_int,
+                       cap_ext);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_tree_scatter, dim3(nb), dim3(256), 0, s, sa, bcnt, boff, C, level, B, split);
+    HIPCHK(hipGetLastError());
+    if (quit_next) break;
+  }
+  TreeCounters hc;
+  HIPCHK(hipMemcpyAsync(&hc, C, sizeof(TreeCounters), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  res->n_levels = level + 1;
+  res->n_unknown = hc.n[(level + 1) & 1];
+  res->n_interior = hc.n_int;
+  res->n_exterior = hc.n_ext;
+  res->n_classified = hc.n_classified;
+  res->n_recheck = 0;
+  for (int i = 0; i < 64; ++i) {
+    res->level_n[i] = hc.level_n[i];
+    res->level_interior[i] = hc.level_int[i];
+    res->level_exterior[i] = hc.level_ext[i];
+  }
+  if (hc.overflow) return fail(NIQ_ERR_CAPACITY, "a tree output buffer was too small");
+  return NIQ_OK;
+}
+
+// ---------------------------------------------------------- ray casting
+size_t niq_cast_workspace_bytes(const niq_cast_opts* opts) {
+  const int n_max = opts ? std::max(opts->n_max_step, 1) : 512;
+  size_t b = 256;                                  // work counter
+  b += ((size_t)(n_max + 2) * sizeof(int) + 255) / 256 * 256;
+  b += 8 * sizeof(DevModel);                       // model table
+  return b;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <int KIND, int V, int W>
+niq_status launch_rays_w(const DevModel* models, const RayArgs& a, hipStream_t s) {
+  constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
+  constexpr int R = Planes<KIND, V, 2>::R;
+  if constexpr (!Fits<R, NB, W>::value) {
+    return fail(NIQ_ERR_UNSUPPORTED, "ray range form does not fit in LDS at this width");
+  } else {
+    auto k = k_cast_rays<KIND, V, NB, NW, W>;
+    const long long tiles = (a.n + NB - 1) / NB;
+    const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+    hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, models, a);
+    HIPCHK(hipGetLastError());
+    return NIQ_OK;
+  }
+}
+
+template <int KIND, int V>
+niq_status launch_rays(int wt, const DevModel* models, const RayArgs& a, hipStream_t s) {
+  switch (wt) {
+    case 64: return launch_rays_w<KIND, V, 64>(models, a, s);
+    case 128: return launch_rays_w<KIND, V, 128>(models, a, s);
+    case 256: return launch_rays_w<KIND, V, 256>(models, a, s);
+    case 512: return launch_rays_w<KIND, V, 512>(models, a, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
+
+long long next_bucket(long long s) {
+  for (int e = 7; e <= 30; ++e)
+    if (s <= (1LL << e)) return 1LL << e;
+  return -1;
+}
+
+}  // namespace
+
+extern "C" niq_status niq_cast_rays(const niq_model* const* models, int n_models, const niq_eval_cfg* cfg,
+                                    const float* roots, const float* dirs, int64_t n, const niq_cast_opts* o,
+                                    float* out_t, int32_t* out_hit_id, int32_t* out_count, void* ws,
+                                    size_t ws_bytes, int64_t* n_evals, void* stream) {
+  if (!models || n_models < 1 || n_models > 8) return fail(NIQ_ERR_INVALID, "need 1..8 models");
+  if (!o) return fail(NIQ_ERR_INVALID, "null opts");
+  if (o->n_substeps < 1 || o->n_max_step < 1) return fail(NIQ_ERR_INVALID, "n_substeps and n_max_step must be >= 1");
+  if (n < 0 || n > (1LL << 31) - 1) return fail(NIQ_ERR_INVALID, "ray count out of range");
+  if (n > 0 && (!roots || !dirs || !out_t || !out_hit_id || !out_count)) return fail(NIQ_ERR_INVALID, "bad buffers");
+  const size_t need = niq_cast_workspace_bytes(o);
+  if (!ws || ws_bytes < need) return fail(NIQ_ERR_CAPACITY, "cast workspace too small: need " + std::to_string(need));
+  int wt = 0;
+  for (int i = 0; i < n_models; ++i) {
+    if (niq_status st = check_model(models[i], 3)) return st;
+    wt = std::max(wt, models[i]->wt);
+  }
+  const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
+  if (cfg && cfg->precision != NIQ_PREC_FP32) return fail(NIQ_ERR_UNSUPPORTED, "cast_rays runs in FP32");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* base = static_cast<unsigned char*>(ws);
+  int* counter = reinterpret_cast<int*>(base);
+  int* hist = reinterpret_cast<int*>(base + 256);
+  const size_t hist_bytes = ((size_t)(o->n_max_step + 2) * sizeof(int) + 255) / 256 * 256;
+  DevModel* table = reinterpret_cast<DevModel*>(base + 256 + hist_bytes);
+  std::vector<DevModel> host_table(n_models);
+  for (int i = 0; i < n_models; ++i) host_table[i] = models[i]->host;
+  HIPCHK(hipMemsetAsync(base, 0, 256 + hist_bytes, s));
+  HIPCHK(hipMemcpyAsync(table, host_table.data(), sizeof(DevModel) * n_models, hipMemcpyHostToDevice, s));
+  if (n > 0) {
+    RayArgs a{};
+    a.roots = roots; a.dirs = dirs; a.n = n; a.n_models = n_models;
+    a.hit_eps = o->hit_eps; a.max_dist = o->max_dist; a.safety = o->safety_factor;
+    a.grow = o->interval_grow_fac; a.shrink = o->interval_shrink_fac; a.init_size = o->init_step;
+    a.n_max_step = o->n_max_step; a.n_substeps = o->n_substeps;
+    a.out_t = out_t; a.out_hit = out_hit_id; a.out_count = out_count;
+    a.next_ray = counter; a.iter_hist = hist;
+    niq_status st;
+    if (mode == NIQ_MODE_AFFINE_FIXED) st = launch_rays<0, 1>(wt, table, a, s);
+    else if (mode == NIQ_MODE_INTERVAL) st = launch_rays<0, 0>(wt, table, a, s);
+    else if (mode == NIQ_MODE_SLOPE_INTERVAL) st = launch_rays<1, 1>(wt, table, a, s);
+    else return fail(NIQ_ERR_UNSUPPORTED, "cast_rays supports interval, affine_fixed and slope_interval");
+    if (st) return st;
+  }
+  std::vector<int> h(o->n_max_step + 2, 0);
+  HIPCHK(hipMemcpyAsync(h.data(), hist, sizeof(int) * h.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  // Reproduce the reference's N_evals: each lock-step iteration evaluates the
+  // whole (bucketed) working array n_substeps times (src/queries.py:164,170-173).
+  if (n_evals) {
+    long long evals = 0, bucket = n, live = n;
+    for (int it = 1; it <= o->n_max_step + 1 && live > 0; ++it) {
+      evals += bucket * o->n_substeps;
+      live -= h[it];
+      if (live > 0) {
+        const long long nbkt = next_bucket(live);
+        if (nbkt > 0 && nbkt < bucket) bucket = nbkt;
+      }
+    }
+    *n_evals = evals;
+  }
+  return NIQ_OK;
+}

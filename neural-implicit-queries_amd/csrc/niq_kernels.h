@@ -1,0 +1,432 @@
+// Kernels of the range-analysis hot path: box classification, point
+// evaluation, the device-resident level-set tree step and the persistent
+// ray caster.  Everything here is gfx950-only HIP; host dispatch lives in
+// niq_api.hip.
+#pragma once
+#include "niq_device.h"
+
+// Workgroup shapes per padded max width: boxes per tile (NB) and waves (NW).
+template <int WMAX> struct TileCfg;
+template <> struct TileCfg<64>  { static constexpr int NB = 32, NW = 4, NBP = 64; };
+template <> struct TileCfg<128> { static constexpr int NB = 32, NW = 4, NBP = 64; };
+template <> struct TileCfg<256> { static constexpr int NB = 16, NW = 8, NBP = 64; };
+template <> struct TileCfg<512> { static constexpr int NB = 16, NW = 8, NBP = 64; };
+
+constexpr int kLdsLimit = 160 * 1024;
+
+template <int R, int NB, int WMAX>
+struct Fits { static constexpr bool value = (size_t)R * NB * WMAX * 4 <= (size_t)kLdsLimit; };
+
+// ----------------------------------------------------------- classification
+struct ClassifyArgs {
+  const float* lower;      // axis-aligned boxes (n, d) or null
+  const float* upper;
+  const float* center;     // general boxes (n, d)
+  const float* vecs;       // (n, vin, d)
+  int vin;                 // vectors per general box
+  int d;                   // spatial dimension (<= 16)
+  const int* n_dev;        // device-resident count (null: use n)
+  long long n;
+  float offset;
+  int* codes;              // (n,) sign codes
+  float* bounds;           // (n, 2) lower/upper, nullable
+  float recheck_tol;       // > 0: list boxes within tol*scale of the threshold
+  int* recheck_idx;        // (recheck_cap,)
+  int* recheck_cnt;        // device counter
+  int recheck_cap;
+};
+
+// Fill the layer-0 input planes of one tile (features 0..15, zero padded).
+template <int KIND, int V, int NB, int NT, int WMAX>
+__device__ __forceinline__ void fill_box_inputs(float* X, const ClassifyArgs& a, long long i0, long long n) {
+  using PL = Planes<KIND, V, 0>;
+  constexpr int RB = PL::RB;
+  for (int e = threadIdx.x; e < RB * NB * 16; e += NT) {
+    const int p = e / (NB * 16);
+    const int b = (e / 16) % NB;
+    const int k = e & 15;
+    const long long i = i0 + b;
+    float val = 0.f;
+    if (i < n && k < a.d) {
+      float c, h = 0.f;
+      if (a.lower) {
+        const float lo = a.lower[i * a.d + k], up = a.upper[i * a.d + k];
+        c = 0.5f * (lo + up);
+        h = up - c;
+        if (p == 0) val = c;
+        else if (KIND == 0 && V == 0) val = fabsf(h);              // interval err
+        else if (p >= 1 && p <= V) val = (p - 1 == k) ? h : 0.f;   // diag(half)
+        else val = 0.f;
+      } else {
+        if (p == 0) val = a.center[i * a.d + k];
+        else if (KIND == 0 && V == 0) {
+          float s = 0.f;
+          for (int r = 0; r < a.vin; ++r) s = s + fabsf(a.vecs[(i * a.vin + r) * a.d + k]);
+          val = s;
+        } else if (p >= 1 && p <= V) val = a.vecs[(i * a.vin + (p - 1)) * a.d + k];
+        else val = 0.f;
+      }
+    }
+    X[xoff<NB, WMAX>(p, b, k)] = val;
+  }
+}
+
+template <int KIND, int V, int NB, int NW, int WMAX>
+__global__ void __launch_bounds__(NW * 64)
+k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
+  using PL = Planes<KIND, V, 0>;
+  constexpr int NT = NW * 64;
+  __shared__ float X[PL::R * NB * WMAX];
+  const long long n = a.n_dev ? (long long)(*a.n_dev) : a.n;
+  const long long ntiles = (n + NB - 1) / NB;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long i0 = tile * NB;
+    fill_box_inputs<KIND, V, NB, NT, WMAX>(X, a, i0, n);
+    mlp_forward<PL, KIND, V, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+      const long long i = i0 + b;
+      if (i >= n) return;
+      float lo, hi, scale;
+      final_bounds<KIND, V>(v, lo, hi, scale);
+      a.codes[i] = sign_code(lo, hi, a.offset);
+      if (a.bounds) { a.bounds[2 * i] = lo; a.bounds[2 * i + 1] = hi; }
+      if (a.recheck_tol > 0.f) {
+        const float tol = a.recheck_tol * scale;
+        if (fabsf(lo - a.offset) <= tol || fabsf(hi + a.offset) <= tol) {
+          const int slot = atomicAdd(a.recheck_cnt, 1);
+          if (slot < a.recheck_cap) a.recheck_idx[slot] = (int)i;
+        }
+      }
+    });
+  }
+}
+
+// ------------------------------------------------------------ point values
+template <int NB, int NW, int WMAX>
+__global__ void __launch_bounds__(NW * 64)
+k_points(const DevModel* __restrict__ Mp, const float* __restrict__ x, int d, long long n,
+         float* __restrict__ out) {
+  using PL = Planes<2, 0, 1>;
+  constexpr int NT = NW * 64;
+  __shared__ float X[NB * WMAX];
+  const long long ntiles = (n + NB - 1) / NB;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long i0 = tile * NB;
+    for (int e = threadIdx.x; e < NB * 16; e += NT) {
+      const int b = e >> 4, k = e & 15;
+      const long long i = i0 + b;
+      X[xoff<NB, WMAX>(0, b, k)] = (i < n && k < d) ? x[i * d + k] : 0.f;
+    }
+    mlp_forward<PL, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+      const long long i = i0 + b;
+      if (i < n) out[i] = v[0];
+    });
+  }
+}
+
+// -------------------------------------------------- level-set tree step
+// Device counters of one tree build.  n[l & 1] is the node count of level l.
+struct TreeCounters {
+  int n[2];
+  int n_int, n_ext;
+  long long n_classified;
+  int overflow;
+  int pad_;
+  int level_n[64];
+  int level_int[64];
+  int level_ext[64];
+};
+
+// Per batch of B nodes: counts of UNKNOWN / POSITIVE / NEGATIVE codes.
+__global__ void __launch_bounds__(256)
+k_tree_count(const int* __restrict__ codes, const TreeCounters* __restrict__ C, int level, int B,
+             int* __restrict__ bcnt) {
+  const int n = C->n[level & 1];
+  const long long start = (long long)blockIdx.x * B;
+  if (start >= n) return;
+  const long long end = min(start + (long long)B, (long long)n);
+  int cu = 0, cp = 0, cn = 0;
+  for (long long i = start + threadIdx.x; i < end; i += blockDim.x) {
+    const int c = codes[i];
+    cu += (c == CODE_UNKNOWN); cp += (c == CODE_POSITIVE); cn += (c == CODE_NEGATIVE);
+  }
+  __shared__ int s[3][256];
+  s[0][threadIdx.x] = cu; s[1][threadIdx.x] = cp; s[2][threadIdx.x] = cn;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + w];
+      s[1][threadIdx.x] += s[1][threadIdx.x + w];
+      s[2][threadIdx.x] += s[2][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bcnt[blockIdx.x * 4 + 0] = s[0][0];
+    bcnt[blockIdx.x * 4 + 1] = s[1][0];
+    bcnt[blockIdx.x * 4 + 2] = s[2][0];
+  }
+}
+
+// Single workgroup: exclusive scan of the per-batch counts, level totals.
+// boff[b] = {unknown prefix, exterior offset, interior offset}.
+__global__ void __launch_bounds__(1024)
+k_tree_scan(const int* __restrict__ bcnt, int* __restrict__ boff, TreeCounters* C, int level, int B,
+            int split, long long cap_next, long long cap_int, long long cap_ext) {
+  const int n = C->n[level & 1];
+  const int nb = (int)((n + (long long)B - 1) / B);
+  __shared__ int s[3][1024];
+  __shared__ int carry[3];
+  if (threadIdx.x == 0) { carry[0] = 0; carry[1] = C->n_ext; carry[2] = C->n_int; }
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    const int b = base + threadIdx.x;
+    int v0 = 0, v1 = 0, v2 = 0;
+    if (b < nb) { v0 = bcnt[b * 4 + 0]; v1 = bcnt[b * 4 + 1]; v2 = bcnt[b * 4 + 2]; }
+    s[0][threadIdx.x] = v0; s[1][threadIdx.x] = v1; s[2][threadIdx.x] = v2;
+    __syncthreads();
+    for (int w = 1; w < 1024; w <<= 1) {   // Hillis-Steele inclusive scan
+      int a0 = 0, a1 = 0, a2 = 0;
+      if ((int)threadIdx.x >= w) {
+        a0 = s[0][threadIdx.x - w]; a1 = s[1][threadIdx.x - w]; a2 = s[2][threadIdx.x - w];
+      }
+      __syncthreads();
+      s[0][threadIdx.x] += a0; s[1][threadIdx.x] += a1; s[2][threadIdx.x] += a2;
+      __syncthreads();
+    }
+    if (b < nb) {
+      boff[b * 4 + 0] = carry[0] + s[0][threadIdx.x] - v0;
+      boff[b * 4 + 1] = carry[1] + s[1][threadIdx.x] - v1;
+      boff[b * 4 + 2] = carry[2] + s[2][threadIdx.x] - v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      carry[0] += s[0][1023]; carry[1] += s[1][1023]; carry[2] += s[2][1023];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int U = carry[0];
+    C->level_n[level] = n;
+    C->level_int[level] = C->n_int;
+    C->level_ext[level] = C->n_ext;
+    C->n_classified += n;
+    C->n_ext = carry[1];
+    C->n_int = carry[2];
+    const long long nn = split ? 2LL * U : (long long)U;
+    if (nn > cap_next || carry[1] > cap_ext || carry[2] > cap_int) C->overflow = 1;
+    C->n[(level + 1) & 1] = (int)min(nn, cap_next);
+  }
+}
+
+struct TreeScatterArgs {
+  const int* codes;
+  const float* lower; const float* upper;   // current level (n, 3)
+  float* next_lower; float* next_upper;     // next level / final unknown list
+  long long cap_next;
+  float* int_lower; float* int_upper; long long cap_int;   // nullable
+  float* ext_lower; float* ext_upper; long long cap_ext;   // nullable
+};
+
+__device__ __forceinline__ void store3(float* p, long long i, float x, float y, float z) {
+  p[3 * i] = x; p[3 * i + 1] = y; p[3 * i + 2] = z;
+}
+
+// Stable per-batch placement of children and finished leaves, in the
+// reference's order (src/kd_tree.py:46-96, src/bucketing.py:16-32).
+__global__ void __launch_bounds__(256)
+k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __restrict__ boff,
+               const TreeCounters* __restrict__ C, int level, int B, int split) {
+  const int n = C->n[level & 1];
+  const long long start = (long long)blockIdx.x * B;
+  if (start >= n) return;
+  const long long end = min(start + (long long)B, (long long)n);
+  const int cnt = (int)(end - start);
+  const int per = (cnt + 255) / 256;
+  const int my0 = threadIdx.x * per;
+  const int my1 = min(my0 + per, cnt);
+  int lu = 0, lp = 0, ln = 0;
+  for (int j = my0; j < my1; ++j) {
+    const int c = a.codes[start + j];
+    lu += (c == CODE_UNKNOWN); lp += (c == CODE_POSITIVE); ln += (c == CODE_NEGATIVE);
+  }
+  __shared__ int s[3][256];
+  s[0][threadIdx.x] = lu; s[1][threadIdx.x] = lp; s[2][threadIdx.x] = ln;
+  __syncthreads();
+  for (int w = 1; w < 256; w <<= 1) {
+    int a0 = 0, a1 = 0, a2 = 0;
+    if ((int)threadIdx.x >= w) { a0 = s[0][threadIdx.x - w]; a1 = s[1][threadIdx.x - w]; a2 = s[2][threadIdx.x - w]; }
+    __syncthreads();
+    s[0][threadIdx.x] += a0; s[1][threadIdx.x] += a1; s[2][threadIdx.x] += a2;
+    __syncthreads();
+  }
+  int ru = s[0][threadIdx.x] - lu, rp = s[1][threadIdx.x] - lp, rn = s[2][threadIdx.x] - ln;
+  const long long Ub = boff[blockIdx.x * 4 + 0];
+  const long long nUb = bcnt[blockIdx.x * 4 + 0];
+  const long long Eb = boff[blockIdx.x * 4 + 1];
+  const long long Ib = boff[blockIdx.x * 4 + 2];
+  for (int j = my0; j < my1; ++j) {
+    const long long i = start + j;
+    const int c = a.codes[i];
+    const float l0 = a.lower[3 * i], l1 = a.lower[3 * i + 1], l2 = a.lower[3 * i + 2];
+    const float u0 = a.upper[3 * i], u1 = a.upper[3 * i + 1], u2 = a.upper[3 * i + 2];
+    if (c == CODE_UNKNOWN) {
+      if (split) {
+        const float e0 = u0 - l0, e1 = u1 - l1, e2 = u2 - l2;
+        int dim = 0; float em = e0;
+        if (e1 > em) { dim = 1; em = e1; }
+        if (e2 > em) { dim = 2; }
+        const float m0 = 0.5f * (l0 + u0), m1 = 0.5f * (l1 + u1), m2 = 0.5f * (l2 + u2);
+        const long long pa = 2 * Ub + ru, pb = 2 * Ub + nUb + ru;
+        if (pa < a.cap_next) {
+          store3(a.next_lower, pa, l0, l1, l2);
+          store3(a.next_upper, pa, dim == 0 ? m0 : u0, dim == 1 ? m1 : u1, dim == 2 ? m2 : u2);
+        }
+        if (pb < a.cap_next) {
+          store3(a.next_lower, pb, dim == 0 ? m0 : l0, dim == 1 ? m1 : l1, dim == 2 ? m2 : l2);
+          store3(a.next_upper, pb, u0, u1, u2);
+        }
+      } else {
+        const long long p = Ub + ru;
+        if (p < a.cap_next) { store3(a.next_lower, p, l0, l1, l2); store3(a.next_upper, p, u0, u1, u2); }
+      }
+      ++ru;
+    } else if (c == CODE_POSITIVE) {
+      const long long p = Eb + rp;
+      if (a.ext_lower && p < a.cap_ext) { store3(a.ext_lower, p, l0, l1, l2); store3(a.ext_upper, p, u0, u1, u2); }
+      ++rp;
+    } else {
+      const long long p = Ib + rn;
+      if (a.int_lower && p < a.cap_int) { store3(a.int_lower, p, l0, l1, l2); store3(a.int_upper, p, u0, u1, u2); }
+      ++rn;
+    }
+  }
+}
+
+// --------------------------------------------------------- ray casting
+struct RayArgs {
+  const float* roots;      // (n, 3)
+  const float* dirs;       // (n, 3)
+  long long n;
+  int n_models;
+  float hit_eps, max_dist, safety, grow, shrink, init_size;
+  int n_max_step, n_substeps;
+  float* out_t; int* out_hit; int* out_count;
+  int* next_ray;           // work counter, zero on entry
+  int* iter_hist;          // (n_max_step + 2,) rays by iterations lived, nullable
+};
+
+// Persistent certified-step ray caster (src/queries.py:39-175).  A workgroup
+// owns NB ray slots; every step it bounds the next segment of each live ray
+// and samples the two points of the hit test, for each function in turn.
+// Terminated slots are refilled from a global counter, so the tile stays full
+// and no host round trip happens until all rays are done.  Slot state is
+// replicated in lanes 0..NB-1 of every wave (each wave computes the same
+// update), and per-slot results are exchanged through LDS.
+template <int KIND, int V, int NB, int NW, int WMAX>
+__global__ void __launch_bounds__(NW * 64)
+k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
+  using PL = Planes<KIND, V, 2>;
+  constexpr int R = PL::R;
+  constexpr int RB = PL::RB;
+  constexpr int NT = NW * 64;
+  __shared__ float X[R * NB * WMAX];
+  int* Xi = reinterpret_cast<int*>(X);
+  const int lane = threadIdx.x & 63;
+  const bool slot_lane = lane < NB;
+
+  int rid = -1, cnt = 0, iters = 0;
+  float t = 0.f, step = 0.f;
+  float ro[3] = {0.f, 0.f, 0.f}, di[3] = {0.f, 0.f, 0.f};
+
+  while (true) {
+    // ---- refill dead slots (identical in every wave)
+    const unsigned long long dead = __ballot(slot_lane && rid < 0);
+    const int n_dead = __popcll(dead);
+    __syncthreads();
+    if (threadIdx.x == 0) Xi[0] = n_dead ? atomicAdd(a.next_ray, n_dead) : 0;
+    __syncthreads();
+    const int base = Xi[0];
+    if (slot_lane && rid < 0) {
+      const int rank = __popcll(dead & ((1ull << lane) - 1ull));
+      const long long r = (long long)base + rank;
+      if (r < a.n) {
+        rid = (int)r;
+        t = 0.f; step = a.init_size; cnt = 0; iters = 0;
+        for (int k = 0; k < 3; ++k) { ro[k] = a.roots[3 * r + k]; di[k] = a.dirs[3 * r + k]; }
+      }
+    }
+    const bool any_live = __ballot(slot_lane && rid >= 0) != 0ull;
+    if (!any_live) break;   // uniform: every wave holds the same slot state
+
+    // ---- one iteration = n_substeps substeps (src/queries.py:94-111)
+    bool is_hit = false;
+    int hit_id = 0, nsteps = 0;
+    for (int sub = 0; sub < a.n_substeps; ++sub) {
+      bool can_step = !is_hit;
+      nsteps += is_hit ? 0 : 1;
+      for (int f = 0; f < a.n_models; ++f) {
+        __syncthreads();   // previous readers of X are done
+        for (int e = threadIdx.x; e < R * NB * 16; e += NT) {
+          const int k = e & 15;
+          if (k >= 3) X[xoff<NB, WMAX>(e / (NB * 16), (e / 16) % NB, k)] = 0.f;
+        }
+        if (threadIdx.x < NB) {
+          const int b = threadIdx.x;
+          const bool live = rid >= 0;
+          const float hs = 0.5f * step;
+          for (int k = 0; k < 3; ++k) {
+            const float ps = ro[k] + t * di[k];
+            const float hv = hs * di[k];
+            const float pm = ps + hv;
+            const float pe = ro[k] + (t + a.hit_eps) * di[k];
+            float vals[R];
+            for (int p = 0; p < R; ++p) vals[p] = 0.f;
+            vals[0] = pm;
+            if constexpr (KIND == 0 && V == 0) vals[1] = fabsf(hv);
+            else vals[1] = hv;
+            vals[RB] = ps;
+            vals[RB + 1] = pe;
+#pragma unroll
+            for (int p = 0; p < R; ++p) X[xoff<NB, WMAX>(p, b, k)] = live ? vals[p] : 0.f;
+          }
+        }
+        mlp_forward<PL, KIND, V, NB, NW, WMAX>(models[f], X, [&](const float* v, int b) {
+          float lo, hi, scale;
+          final_bounds<KIND, V>(v, lo, hi, scale);
+          X[4 * b + 0] = __int_as_float(sign_code(lo, hi, 0.f));
+          X[4 * b + 1] = v[RB];
+          X[4 * b + 2] = v[RB + 1];
+        });
+        __syncthreads();
+        if (slot_lane) {
+          const int code = __float_as_int(X[4 * lane]);
+          const float v0 = X[4 * lane + 1], v1 = X[4 * lane + 2];
+          can_step = can_step && (code == CODE_POSITIVE || code == CODE_NEGATIVE);
+          const bool this_hit = sign_f(v0) != sign_f(v1);
+          if (this_hit) hit_id = f + 1;
+          is_hit = is_hit || this_hit;
+        }
+      }
+      const float this_step = can_step ? step : a.hit_eps;
+      if (!is_hit) t = t + this_step * a.safety;
+      step = can_step ? step * a.grow : step * a.shrink;
+      step = fmaxf(step, a.hit_eps);
+    }
+
+    // ---- termination (src/queries.py:113-129)
+    if (slot_lane && rid >= 0) {
+      cnt += nsteps;
+      ++iters;
+      const bool term = is_hit || (t > a.max_dist) || (cnt >= a.n_max_step);
+      if (term) {
+        if (threadIdx.x < 64) {   // wave 0 writes
+          a.out_t[rid] = t;
+          a.out_hit[rid] = hit_id;
+          a.out_count[rid] = cnt;
+          if (a.iter_hist) atomicAdd(&a.iter_hist[min(iters, a.n_max_step + 1)], 1);
+        }
+        rid = -1;
+      }
+    }
+  }
+}

@@ -1,0 +1,224 @@
+"""GPU parity: the gfx950 kernels (through the C ABI) against the oracle.
+
+Bars (written here, checked per case):
+  * plain values and bounds: |gpu - oracle32| <= 1e-5 * (|oracle| + mean |oracle|)
+    (MFMA and BLAS sum in different orders; nothing else differs);
+  * sign codes: identical except cells whose FP64 bound lies within
+    1e-6 * scale of the threshold (scale = |lower| + |upper|), which are counted;
+  * tree lists: identical arrays (order included) when no near-threshold
+    decision differs;
+  * rays: identical (t, hit_id, count) for all but near-threshold rays.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import funcs as ofuncs, tree as otree, rays as orays, mlp as omlp, affine as oaff
+from niq import models, mlp as nmlp, implicit_mlp_utils, kd_tree, queries, render
+
+pytestmark = pytest.mark.gpu
+
+LO = np.array([-1., -1., -1.], np.float32)
+UP = np.array([1., 1., 1.], np.float32)
+NEAR_REL = 1e-6
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def model(name):
+    if name in models.CONFIGS:
+        return models.config_model(name)
+    act = {"small_elu": "elu", "small_sin": "sin", "small_tanh": "tanh", "small_relu": "relu"}[name]
+    return models.random_mlp([48, 40, 24], activation=act, seed=3)
+
+
+def boxes(n, seed, scale):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, size=(n, 3))
+    h = rng.uniform(0.002, 0.25, size=(n, 3)) * scale
+    return (c - h).astype(np.float32), (c + h).astype(np.float32)
+
+
+def rel_close(got, ref, tol=1e-5):
+    scale = np.abs(ref) + np.mean(np.abs(ref))
+    return np.abs(got - ref) <= tol * scale
+
+
+MODELS = ["c1_relu_3x32", "c2_relu_256x6", "c4_relu_512x8", "small_elu", "small_sin", "small_tanh"]
+
+
+@pytest.mark.parametrize("name", MODELS)
+def test_points_match_oracle(name):
+    note(f"[points] {name}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    x = np.random.default_rng(0).uniform(-1, 1, size=(3000, 3)).astype(np.float32)
+    got = func(p, torch.as_tensor(x, device="cuda")).cpu().numpy()
+    ref = omlp.eval_points(p, x, np.float32)
+    assert np.all(rel_close(got, ref)), np.max(np.abs(got - ref))
+
+
+@pytest.mark.parametrize("name", MODELS)
+@pytest.mark.parametrize("mode", ["interval", "affine_fixed", "slope_interval"])
+def test_classify_matches_oracle(name, mode):
+    note(f"[classify] {name} {mode}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, mode)
+    lo, up = boxes(1500 if name != "c4_relu_512x8" else 600, seed=1, scale=0.5)
+    codes, bounds = func.classify_box(p, torch.as_tensor(lo, device="cuda"),
+                                      torch.as_tensor(up, device="cuda"), return_bounds=True)
+    codes = codes.cpu().numpy()
+    bounds = bounds.cpu().numpy()
+    of32 = ofuncs.OracleFunc(p, mode, np.float32)
+    of64 = ofuncs.OracleFunc(p, mode, np.float64)
+    r_lo, r_hi = of32.bounds_box(lo, up)
+    assert np.all(rel_close(bounds[:, 0], r_lo)), np.max(np.abs(bounds[:, 0] - r_lo))
+    assert np.all(rel_close(bounds[:, 1], r_hi)), np.max(np.abs(bounds[:, 1] - r_hi))
+    ref_codes = oaff.sign_code(r_lo, r_hi, 0.0)
+    d_lo, d_hi = of64.bounds_box(lo, up)
+    scale = np.abs(d_lo) + np.abs(d_hi)
+    near = (np.abs(d_lo) <= NEAR_REL * scale) | (np.abs(d_hi) <= NEAR_REL * scale)
+    mism = codes != ref_codes
+    note(f"  {mism.sum()} code mismatches, {near.sum()} near-threshold cells")
+    assert not np.any(mism & ~near)
+    assert (codes != 0).sum() > 0 or mode == "interval"
+
+
+def test_general_boxes_match_oracle():
+    p = model("c2_relu_256x6")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    rng = np.random.default_rng(4)
+    c = rng.uniform(-1, 1, size=(700, 3)).astype(np.float32)
+    v = (rng.normal(size=(700, 2, 3)) * 0.05).astype(np.float32)
+    codes, bounds = func.classify_general_box(p, torch.as_tensor(c, device="cuda"),
+                                              torch.as_tensor(v, device="cuda"), return_bounds=True)
+    r_lo, r_hi = ofuncs.OracleFunc(p, "affine_fixed").bounds_general(c, v)
+    assert np.all(rel_close(bounds.cpu().numpy()[:, 0], r_lo))
+    assert np.all(rel_close(bounds.cpu().numpy()[:, 1], r_hi))
+
+
+@pytest.mark.parametrize("mode", ["affine_fixed", "interval", "slope_interval"])
+def test_tree_c1_matches_oracle(mode):
+    note(f"[tree c1 depth 12] {mode}")
+    p = model("c1_relu_3x32")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, mode)
+    out, st = kd_tree.build_tree(func, p, LO, UP, split_depth=12, with_interior_nodes=True,
+                                 with_exterior_nodes=True)
+    ref = otree.construct_uniform_unknown_levelset_tree(
+        ofuncs.OracleFunc(p, mode), LO, UP, split_depth=12, with_interior_nodes=True,
+        with_exterior_nodes=True)
+    note(f"  gpu {st['n_classified']} boxes, oracle {ref['stats']['n_classified']}")
+    assert st["n_classified"] == ref["stats"]["n_classified"]
+    for key in ("unknown", "interior", "exterior"):
+        v = out[f"{key}_node_valid"].cpu().numpy()
+        assert v.shape == ref[f"{key}_node_valid"].shape
+        np.testing.assert_array_equal(v, ref[f"{key}_node_valid"])
+        np.testing.assert_array_equal(out[f"{key}_node_lower"].cpu().numpy()[v],
+                                      ref[f"{key}_node_lower"][v])
+        np.testing.assert_array_equal(out[f"{key}_node_upper"].cpu().numpy()[v],
+                                      ref[f"{key}_node_upper"][v])
+
+
+def test_tree_batch_size_changes_only_order():
+    p = model("c1_relu_3x32")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    a, _ = kd_tree.build_tree(func, p, LO, UP, split_depth=10, batch_process_size=2048)
+    b, _ = kd_tree.build_tree(func, p, LO, UP, split_depth=10, batch_process_size=64)
+    ref = otree.construct_uniform_unknown_levelset_tree(ofuncs.OracleFunc(p, "affine_fixed"), LO, UP,
+                                                        split_depth=10, batch_process_size=64)
+    v = b["unknown_node_valid"].cpu().numpy()
+    np.testing.assert_array_equal(b["unknown_node_lower"].cpu().numpy()[v], ref["unknown_node_lower"][v])
+    sa = {tuple(r) for r in a["unknown_node_lower"].cpu().numpy()[v]}
+    sb = {tuple(r) for r in b["unknown_node_lower"].cpu().numpy()[v]}
+    assert sa == sb
+
+
+def test_tree_c2_depth18_properties():
+    """Full-size C2 tree: size-independent properties plus a sampled FP64 check."""
+    note("[tree c2 depth 18]")
+    p = model("c2_relu_256x6")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    out, st = kd_tree.build_tree(func, p, LO, UP, split_depth=18, with_interior_nodes=True,
+                                 with_exterior_nodes=True)
+    note(f"  {st['n_classified']} boxes bounded, {st['n_unknown']} unknown leaves")
+    assert st["n_classified"] == sum(st["level_counts"])
+    n = st["n_unknown"]
+    lo = out["unknown_node_lower"][:n]
+    up = out["unknown_node_upper"][:n]
+    w = (up - lo).cpu().numpy()
+    np.testing.assert_array_equal(w, np.full_like(w, 2.0 / 64))
+    # idempotence: every returned leaf re-classifies as UNKNOWN
+    codes = func.classify_box(p, lo, up).cpu().numpy()
+    assert np.all(codes == 0)
+    # sampled no-false-empty check against the FP64 oracle
+    rng = np.random.default_rng(0)
+    of64 = ofuncs.OracleFunc(p, "affine_fixed", np.float64)
+    for key, sign in (("interior", 2), ("exterior", 1)):
+        m = int(out[f"{key}_node_valid"].sum())
+        idx = rng.choice(m, size=min(m, 1500), replace=False)
+        l = out[f"{key}_node_lower"][:m].cpu().numpy()[idx]
+        u = out[f"{key}_node_upper"][:m].cpu().numpy()[idx]
+        c64 = of64.classify_box(l, u)
+        note(f"  {key}: {m} leaves, sampled FP64 disagreements {(c64 != sign).sum()}")
+        assert np.all(c64 == sign)
+
+
+def camera(res, fov=60.0, eye=(0., 0., 3.)):
+    eye = np.array(eye, np.float32)
+    look, up, _ = render.look_at(eye)
+    return render.generate_camera_rays(eye, look, up, res=res, fov_deg=fov, device=None)
+
+
+def test_rays_match_oracle_c1():
+    note("[rays c1 64x64]")
+    p = model("c1_relu_3x32")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    roots, dirs = camera(64)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    t, hid, cnt, ne = queries.cast_rays((func,), (p,), roots, dirs, opts)
+    of = ofuncs.OracleFunc(p, "affine_fixed")
+    rt, rh, rc, rne = orays.cast_rays((of,), roots, dirs, opts)
+    t, hid, cnt = t.cpu().numpy(), hid.cpu().numpy(), cnt.cpu().numpy()
+    same = (t == rt) & (hid == rh) & (cnt == rc)
+    note(f"  {same.mean() * 100:.3f}% rays identical, n_evals gpu {ne} oracle {rne}")
+    assert same.mean() >= 0.995
+    assert (hid == rh).mean() >= 0.998
+
+
+def test_rays_plane_closed_form():
+    nrm = np.array([0.2, 0.3, 1.0], np.float32)
+    nrm /= np.linalg.norm(nrm)
+    A1 = np.concatenate([np.eye(3), -np.eye(3)], 1)
+    A2 = np.concatenate([0.5 * nrm, -0.5 * nrm])[:, None]
+    p = nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-0.25]),
+                         nmlp.squeeze_last()])
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    roots, dirs = camera(48, eye=(0.1, -0.2, 3.0))
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    t, hid, _, _ = queries.cast_rays((func,), (p,), roots, dirs, opts)
+    t, hid = t.cpu().numpy(), hid.cpu().numpy()
+    t_star = (0.25 - roots @ nrm) / (dirs @ nrm)
+    hit = (t_star > 0) & (t_star < 6.0)
+    assert np.all(hid[hit] == 1) and np.all(hid[~hit] == 0)
+    assert np.all(np.abs(t[hit] - t_star[hit]) <= 1.01e-3)
+
+
+def test_rays_two_functions():
+    p1 = model("c1_relu_3x32")
+    p2 = model("small_relu")
+    f1, _ = implicit_mlp_utils.implicit_from_params(p1, "affine_fixed")
+    f2, _ = implicit_mlp_utils.implicit_from_params(p2, "affine_fixed")
+    roots, dirs = camera(32)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    t, hid, cnt, _ = queries.cast_rays((f1, f2), (p1, p2), roots, dirs, opts)
+    rt, rh, rc, _ = orays.cast_rays((ofuncs.OracleFunc(p1), ofuncs.OracleFunc(p2)), roots, dirs, opts)
+    same = (t.cpu().numpy() == rt) & (hid.cpu().numpy() == rh)
+    assert same.mean() >= 0.99
