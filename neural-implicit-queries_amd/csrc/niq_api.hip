@@ -108,11 +108,13 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
 
   // Pack each layer's W (in, out) into MFMA A-fragments: fragment (ft, kb) is
   // 64 lanes x 4 floats, lane l / slot j holding W[16 kb + 4 (l / 16) + j][16 ft + l % 16].
-  std::vector<size_t> woff(n_layers), boff(n_layers);
+  std::vector<size_t> woff(n_layers), poff(n_layers), boff(n_layers);
   size_t total = 0;
   for (int i = 0; i < n_layers; ++i) {
     const size_t ip = round16(dims[i]), op = round16(dims[i + 1]);
     woff[i] = total;
+    total += ip * op * sizeof(float);
+    poff[i] = total;
     total += ip * op * sizeof(float);
     boff[i] = total;
     total += op * sizeof(float);
@@ -133,6 +135,9 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
             const float v = (k < in && f < outd) ? W[i][(size_t)k * outd + f] : 0.f;
             wp[((size_t)(ft * nkb + kb) * 64 + l) * 4 + j] = v;
           }
+    float* pw = reinterpret_cast<float*>(host.data() + poff[i]);
+    for (int k = 0; k < in; ++k)
+      for (int f = 0; f < outd; ++f) pw[(size_t)k * op + f] = W[i][(size_t)k * outd + f];
     float* bp = reinterpret_cast<float*>(host.data() + boff[i]);
     for (int f = 0; f < outd; ++f) bp[f] = (b && b[i]) ? b[i][f] : 0.f;
   }
@@ -161,6 +166,7 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
     L.act = (i + 1 < n_layers) ? acts[i] : ACT_NONE;
     L.Wp = reinterpret_cast<const floatx4*>(static_cast<unsigned char*>(m->weights) + woff[i]);
     L.bias = reinterpret_cast<const float*>(static_cast<unsigned char*>(m->weights) + boff[i]);
+    L.W = reinterpret_cast<const float*>(static_cast<unsigned char*>(m->weights) + poff[i]);
   }
   e = hipMemcpy(m->dev, &m->host, sizeof(DevModel), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -200,6 +206,12 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
     const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
     hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, a);
     HIPCHK(hipGetLastError());
+    if (a.recheck_tol > 0.f) {
+      auto k64 = k_recheck_fp64<KIND, V, W>;
+      const unsigned g64 = grid_for(std::min<long long>(max_n, a.recheck_cap), persistent_blocks(k64, 256));
+      hipLaunchKernelGGL(k64, dim3(g64), dim3(256), 0, s, m->dev, a);
+      HIPCHK(hipGetLastError());
+    }
     return NIQ_OK;
   }
 }
@@ -253,12 +265,35 @@ niq_status launch_points_w(const niq_model* m, const float* x, int d, long long 
   return NIQ_OK;
 }
 
+// Classification with the optional FP64 re-test list (stream-ordered scratch).
+niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, int nvec, ClassifyArgs a,
+                                 long long n, hipStream_t s);
+
 niq_status check_model(const niq_model* m, int d) {
   if (!m) return fail(NIQ_ERR_INVALID, "null model");
   if (d != m->in_dim) return fail(NIQ_ERR_INVALID, "point dimension does not match the model input");
   return NIQ_OK;
 }
 
+}  // namespace
+
+namespace {
+niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, int nvec, ClassifyArgs a,
+                                 long long n, hipStream_t s) {
+  const float tol = cfg ? cfg->recheck_tol : 0.f;
+  if (!(tol > 0.f) || n == 0) return classify_dispatch(m, cfg, nvec, a, n, s);
+  if (n > INT_MAX) return fail(NIQ_ERR_UNSUPPORTED, "FP64 re-test supports at most 2^31-1 boxes per call");
+  void* scratch = nullptr;
+  HIPCHK(hipMallocAsync(&scratch, sizeof(int) * (size_t)(n + 64), s));
+  HIPCHK(hipMemsetAsync(scratch, 0, sizeof(int), s));
+  a.recheck_tol = tol;
+  a.recheck_cnt = static_cast<int*>(scratch);
+  a.recheck_idx = static_cast<int*>(scratch) + 64;
+  a.recheck_cap = (int)n;
+  niq_status st = classify_dispatch(m, cfg, nvec, a, n, s);
+  HIPCHK(hipFreeAsync(scratch, s));
+  return st;
+}
 }  // namespace
 
 extern "C" {
@@ -271,7 +306,7 @@ niq_status niq_classify_boxes(const niq_model* m, const niq_eval_cfg* cfg, const
   ClassifyArgs a{};
   a.lower = lower; a.upper = upper; a.d = d; a.n = n; a.offset = offset;
   a.codes = codes; a.bounds = bounds;
-  return classify_dispatch(m, cfg, d, a, n, (hipStream_t)stream);
+  return classify_with_recheck(m, cfg, d, a, n, (hipStream_t)stream);
 }
 
 niq_status niq_classify_general_boxes(const niq_model* m, const niq_eval_cfg* cfg, const float* center,
@@ -284,7 +319,7 @@ niq_status niq_classify_general_boxes(const niq_model* m, const niq_eval_cfg* cf
   a.center = center; a.vecs = vecs; a.vin = v; a.d = d; a.n = n; a.offset = offset;
   a.codes = codes; a.bounds = bounds;
   const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
-  return classify_dispatch(m, cfg, mode == NIQ_MODE_INTERVAL ? 0 : v, a, n, (hipStream_t)stream);
+  return classify_with_recheck(m, cfg, mode == NIQ_MODE_INTERVAL ? 0 : v, a, n, (hipStream_t)stream);
 }
 
 niq_status niq_eval_points(const niq_model* m, const float* x, int64_t n, int d, float* out, void* stream) {
@@ -306,7 +341,7 @@ namespace {
 struct TreeLayout {
   long long cap;      // max nodes in any level list
   long long nbmax;    // max batches per level
-  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], total;
+  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], off_recheck, total;
 };
 
 niq_status tree_layout(const niq_tree_params* p, TreeLayout* L) {
@@ -334,6 +369,7 @@ niq_status tree_layout(const niq_tree_params* p, TreeLayout* L) {
   L->off_boff = take(sizeof(int) * 4 * L->nbmax);
   L->off_lvl[0] = take(sizeof(float) * 6 * cap);
   L->off_lvl[1] = take(sizeof(float) * 6 * cap);
+  L->off_recheck = take(sizeof(int) * cap);
   L->total = o;
   return NIQ_OK;
 }
@@ -342,6 +378,7 @@ __global__ void k_tree_init(TreeCounters* C, float* lo, float* up, float l0, flo
                             float u1, float u2) {
   if (threadIdx.x == 0) {
     C->n[0] = 1; C->n[1] = 0; C->n_int = 0; C->n_ext = 0; C->n_classified = 0; C->overflow = 0;
+    C->recheck_cnt = 0; C->n_recheck = 0;
     lo[0] = l0; lo[1] = l1; lo[2] = l2;
     up[0] = u0; up[1] = u1; up[2] = u2;
   }
@@ -405,6 +442,12 @@ niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq
     ClassifyArgs a{};
     a.lower = cur_lo; a.upper = cur_up; a.d = 3; a.n_dev = &C->n[level & 1]; a.offset = p->offset;
     a.codes = codes;
+    if (cfg && cfg->recheck_tol > 0.f) {
+      a.recheck_tol = cfg->recheck_tol;
+      a.recheck_cnt = &C->recheck_cnt;
+      a.recheck_idx = reinterpret_cast<int*>(base + L.off_recheck);
+      a.recheck_cap = (int)std::min<long long>(cap, INT_MAX);
+    }
     if (niq_status st = classify_dispatch(m, cfg, 3, a, max_n, s)) return st;
 
     const unsigned nb = (unsigned)((max_n + B - 1) / B);
@@ -441,7 +484,7 @@ niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq
   res->n_interior = hc.n_int;
   res->n_exterior = hc.n_ext;
   res->n_classified = hc.n_classified;
-  res->n_recheck = 0;
+  res->n_recheck = hc.n_recheck;
   for (int i = 0; i < 64; ++i) {
     res->level_n[i] = hc.level_n[i];
     res->level_interior[i] = hc.level_int[i];

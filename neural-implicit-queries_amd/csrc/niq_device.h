@@ -33,6 +33,7 @@ struct DevLayer {
   int pad_;
   const floatx4* Wp;     // fragment-packed weights, [out_pad/16][in_pad/16][64 lanes]
   const float* bias;     // [out_pad]
+  const float* W;        // row-major (in_pad, out_pad) copy for the FP64 re-test
 };
 
 struct DevModel {
@@ -44,130 +45,178 @@ struct DevModel {
 };
 
 // ------------------------------------------------------------------ helpers
-__device__ __forceinline__ float relu_f(float x) { return x > 0.f ? x : 0.f; }
+// Math by element type, so each rule is written once for FP32 (the kernels)
+// and FP64 (the near-threshold re-test).
+template <class T> struct Fm;
+template <> struct Fm<float> {
+  static __device__ __forceinline__ float exp(float x) { return expf(x); }
+  static __device__ __forceinline__ float expm1(float x) { return expm1f(x); }
+  static __device__ __forceinline__ float log(float x) { return logf(x); }
+  static __device__ __forceinline__ float sin(float x) { return sinf(x); }
+  static __device__ __forceinline__ float acos(float x) { return acosf(x); }
+  static __device__ __forceinline__ float tanh(float x) { return tanhf(x); }
+  static __device__ __forceinline__ float atanh(float x) { return atanhf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float ceil(float x) { return ceilf(x); }
+  static __device__ __forceinline__ float floor(float x) { return floorf(x); }
+  static __device__ __forceinline__ float abs(float x) { return fabsf(x); }
+  static __device__ __forceinline__ float min(float a, float b) { return fminf(a, b); }
+  static __device__ __forceinline__ float max(float a, float b) { return fmaxf(a, b); }
+  static constexpr float kMax = 3.4028234663852886e38f;
+  static constexpr float kTwoPi = 6.2831854820251465f;    // float32(2 pi)
+  static constexpr float kHalfPi = 1.5707963705062866f;   // float32(pi / 2)
+};
+template <> struct Fm<double> {
+  static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+  static __device__ __forceinline__ double expm1(double x) { return ::expm1(x); }
+  static __device__ __forceinline__ double log(double x) { return ::log(x); }
+  static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
+  static __device__ __forceinline__ double acos(double x) { return ::acos(x); }
+  static __device__ __forceinline__ double tanh(double x) { return ::tanh(x); }
+  static __device__ __forceinline__ double atanh(double x) { return ::atanh(x); }
+  static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+  static __device__ __forceinline__ double ceil(double x) { return ::ceil(x); }
+  static __device__ __forceinline__ double floor(double x) { return ::floor(x); }
+  static __device__ __forceinline__ double abs(double x) { return ::fabs(x); }
+  static __device__ __forceinline__ double min(double a, double b) { return ::fmin(a, b); }
+  static __device__ __forceinline__ double max(double a, double b) { return ::fmax(a, b); }
+  static constexpr double kMax = 1.7976931348623157e308;
+  static constexpr double kTwoPi = 6.283185307179586;
+  static constexpr double kHalfPi = 1.5707963267948966;
+};
 
-// jnp.nan_to_num(x, nan=0): NaN -> 0, +-inf -> +-FLT_MAX
-__device__ __forceinline__ float nan_to_num_f(float x) {
-  if (x != x) return 0.f;
-  if (isinf(x)) return x > 0.f ? 3.4028234663852886e38f : -3.4028234663852886e38f;
+template <class T> __device__ __forceinline__ T relu_t(T x) { return x > T(0) ? x : T(0); }
+
+// jnp.nan_to_num(x, nan=0): NaN -> 0, +-inf -> +-max finite
+template <class T> __device__ __forceinline__ T nan_to_num_t(T x) {
+  if (x != x) return T(0);
+  if (isinf(x)) return x > T(0) ? Fm<T>::kMax : -Fm<T>::kMax;
   return x;
 }
-__device__ __forceinline__ float clipf(float x, float lo, float hi) {
-  return fminf(fmaxf(x, lo), hi);
+template <class T> __device__ __forceinline__ T clip_t(T x, T lo, T hi) {
+  return Fm<T>::min(Fm<T>::max(x, lo), hi);
 }
-__device__ __forceinline__ float elu_f(float x) { return x > 0.f ? x : expm1f(fminf(x, 0.f)); }
+template <class T> __device__ __forceinline__ T elu_t(T x) {
+  return x > T(0) ? x : Fm<T>::expm1(Fm<T>::min(x, T(0)));
+}
 
-__device__ __forceinline__ float act_point(int act, float x) {
+template <class T> __device__ __forceinline__ T act_point(int act, T x) {
   switch (act) {
-    case ACT_RELU: return relu_f(x);
-    case ACT_ELU: return elu_f(x);
-    case ACT_SIN: return sinf(x);
-    case ACT_TANH: return tanhf(x);
+    case ACT_RELU: return relu_t(x);
+    case ACT_ELU: return elu_t(x);
+    case ACT_SIN: return Fm<T>::sin(x);
+    case ACT_TANH: return Fm<T>::tanh(x);
     default: return x;
   }
 }
 
-// ------------------------------------------------ linearisation rules (FP32)
+// ------------------------------------------------ linearisation rules
 // Each returns alpha, beta, delta for the output range [lo, hi] of one
 // feature, with |act(x) - (alpha x + beta)| <= delta on [lo, hi].
 
 // Chebyshev ReLU, src/affine_layers.py:34-56
-__device__ __forceinline__ void relu_rule(float lo, float hi, float& a, float& b, float& d) {
-  float al = (relu_f(hi) - relu_f(lo)) / (hi - lo);
-  if (lo >= 0.f) al = 1.f;
-  if (hi < 0.f) al = 0.f;
-  al = clipf(nan_to_num_f(al), 0.f, 1.f);
-  float be = (relu_f(lo) - al * lo) / 2.f;
+template <class T>
+__device__ __forceinline__ void relu_rule(T lo, T hi, T& a, T& b, T& d) {
+  T al = (relu_t(hi) - relu_t(lo)) / (hi - lo);
+  if (lo >= T(0)) al = T(1);
+  if (hi < T(0)) al = T(0);
+  al = clip_t(nan_to_num_t(al), T(0), T(1));
+  T be = (relu_t(lo) - al * lo) / T(2);
   a = al; b = be; d = be;
 }
 
 // src/affine_layers.py:59-98
-__device__ __forceinline__ void elu_rule(float lo, float hi, float& a, float& b, float& d) {
-  float lF = elu_f(lo), uF = elu_f(hi);
-  float lS = fminf(expf(lo), 1.f), uS = fminf(expf(hi), 1.f);
-  float al = (uF - lF) / (hi - lo);
-  if (lo >= 0.f) al = 1.f;
-  al = nan_to_num_f(al);
-  al = fminf(fmaxf(al, lS), uS);
-  float r_up = lF - al * lo;
-  float xl = fminf(fmaxf(logf(al), lo), hi);
-  float r_lo = (al - 1.f) - al * xl;
-  float be = 0.5f * (r_up + r_lo);
-  float de = 0.5f * fabsf(r_up - r_lo);
-  if (lo >= 0.f) { al = 1.f; be = 0.f; de = 0.f; }
+template <class T>
+__device__ __forceinline__ void elu_rule(T lo, T hi, T& a, T& b, T& d) {
+  T lF = elu_t(lo), uF = elu_t(hi);
+  T lS = Fm<T>::min(Fm<T>::exp(lo), T(1)), uS = Fm<T>::min(Fm<T>::exp(hi), T(1));
+  T al = (uF - lF) / (hi - lo);
+  if (lo >= T(0)) al = T(1);
+  al = nan_to_num_t(al);
+  al = Fm<T>::min(Fm<T>::max(al, lS), uS);
+  T r_up = lF - al * lo;
+  T xl = Fm<T>::min(Fm<T>::max(Fm<T>::log(al), lo), hi);
+  T r_lo = (al - T(1)) - al * xl;
+  T be = T(0.5) * (r_up + r_lo);
+  T de = T(0.5) * Fm<T>::abs(r_up - r_lo);
+  if (lo >= T(0)) { al = T(1); be = T(0); de = T(0); }
   a = al; b = be; d = de;
 }
 
 // src/utils.py:209-231
-__device__ __forceinline__ void sin_bound_f(float lo, float hi, float& ol, float& ou) {
-  const float two_pi = 6.2831854820251465f;   // float32(2 pi)
-  float fl = sinf(lo), fu = sinf(hi);
-  float l2 = lo / two_pi, u2 = hi / two_pi;
-  bool cmin = ceilf(l2 - 0.75f) < (u2 - 0.75f);
-  bool cmax = ceilf(l2 - 0.25f) < (u2 - 0.25f);
-  ol = cmin ? -1.f : fminf(fl, fu);
-  ou = cmax ? 1.f : fmaxf(fl, fu);
+template <class T>
+__device__ __forceinline__ void sin_bound_t(T lo, T hi, T& ol, T& ou) {
+  T fl = Fm<T>::sin(lo), fu = Fm<T>::sin(hi);
+  T l2 = lo / Fm<T>::kTwoPi, u2 = hi / Fm<T>::kTwoPi;
+  bool cmin = Fm<T>::ceil(l2 - T(0.75)) < (u2 - T(0.75));
+  bool cmax = Fm<T>::ceil(l2 - T(0.25)) < (u2 - T(0.25));
+  ol = cmin ? T(-1) : Fm<T>::min(fl, fu);
+  ou = cmax ? T(1) : Fm<T>::max(fl, fu);
 }
-__device__ __forceinline__ void cos_bound_f(float lo, float hi, float& ol, float& ou) {
-  const float half_pi = 1.5707963705062866f;  // float32(pi / 2)
-  sin_bound_f(lo + half_pi, hi + half_pi, ol, ou);
+template <class T>
+__device__ __forceinline__ void cos_bound_t(T lo, T hi, T& ol, T& ou) {
+  sin_bound_t(lo + Fm<T>::kHalfPi, hi + Fm<T>::kHalfPi, ol, ou);
 }
 
 // src/affine_layers.py:100-138
-__device__ __forceinline__ void sin_rule(float lo, float hi, float& a, float& b, float& d) {
-  const float two_pi = 6.2831854820251465f;
-  float sl, su;
-  cos_bound_f(lo, hi, sl, su);
-  float al = clipf(0.5f * (sl + su), -1.f, 1.f);
-  float iA = acosf(al), iB = -iA;
-  float locs[6];
+template <class T>
+__device__ __forceinline__ void sin_rule(T lo, T hi, T& a, T& b, T& d) {
+  const T two_pi = Fm<T>::kTwoPi;
+  T sl, su;
+  cos_bound_t(lo, hi, sl, su);
+  T al = clip_t(T(0.5) * (sl + su), T(-1), T(1));
+  T iA = Fm<T>::acos(al), iB = -iA;
+  T locs[6];
   locs[0] = lo; locs[1] = hi;
-  locs[2] = two_pi * ceilf((lo + iA) / two_pi) - iA;
-  locs[3] = two_pi * floorf((hi - iA) / two_pi) + iA;
-  locs[4] = two_pi * ceilf((lo + iB) / two_pi) - iB;
-  locs[5] = two_pi * floorf((hi - iB) / two_pi) + iB;
-  float rl = INFINITY, ru = -INFINITY;
+  locs[2] = two_pi * Fm<T>::ceil((lo + iA) / two_pi) - iA;
+  locs[3] = two_pi * Fm<T>::floor((hi - iA) / two_pi) + iA;
+  locs[4] = two_pi * Fm<T>::ceil((lo + iB) / two_pi) - iB;
+  locs[5] = two_pi * Fm<T>::floor((hi - iB) / two_pi) + iB;
+  T rl = T(INFINITY), ru = T(-INFINITY);
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    float x = fminf(fmaxf(locs[i], lo), hi);
-    float v = sinf(x) - al * x;
-    rl = fminf(rl, v); ru = fmaxf(ru, v);
+    T x = Fm<T>::min(Fm<T>::max(locs[i], lo), hi);
+    T v = Fm<T>::sin(x) - al * x;
+    rl = Fm<T>::min(rl, v); ru = Fm<T>::max(ru, v);
   }
-  float be = 0.5f * (ru + rl);
+  T be = T(0.5) * (ru + rl);
   a = al; b = be; d = ru - be;
 }
 
 // TanH (this project's rule; the reference has none): secant slope, extrema
 // of tanh(x) - alpha x at the endpoints and +-atanh(sqrt(1 - alpha)).
-__device__ __forceinline__ void tanh_rule(float lo, float hi, float& a, float& b, float& d) {
-  float tl = tanhf(lo), tu = tanhf(hi);
-  float al = (tu - tl) / (hi - lo);
-  float dl = 1.f - tl * tl;
+template <class T>
+__device__ __forceinline__ void tanh_rule(T lo, T hi, T& a, T& b, T& d) {
+  T tl = Fm<T>::tanh(lo), tu = Fm<T>::tanh(hi);
+  T al = (tu - tl) / (hi - lo);
+  T dl = T(1) - tl * tl;
   al = (hi > lo) ? al : dl;
-  al = clipf(nan_to_num_f(al), 0.f, 1.f);
-  float xs = atanhf(sqrtf(1.f - al));
-  float locs[4] = {lo, hi, xs, -xs};
-  float rl = INFINITY, ru = -INFINITY;
+  al = clip_t(nan_to_num_t(al), T(0), T(1));
+  T xs = Fm<T>::atanh(Fm<T>::sqrt(T(1) - al));
+  T locs[4] = {lo, hi, xs, -xs};
+  T rl = T(INFINITY), ru = T(-INFINITY);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    float x = fminf(fmaxf(locs[i], lo), hi);
-    float v = tanhf(x) - al * x;
-    rl = fminf(rl, v); ru = fmaxf(ru, v);
+    T x = Fm<T>::min(Fm<T>::max(locs[i], lo), hi);
+    T v = Fm<T>::tanh(x) - al * x;
+    rl = Fm<T>::min(rl, v); ru = Fm<T>::max(ru, v);
   }
-  a = al; b = 0.5f * (ru + rl); d = 0.5f * (ru - rl);
+  a = al; b = T(0.5) * (ru + rl); d = T(0.5) * (ru - rl);
 }
 
-__device__ __forceinline__ void act_rule(int act, float lo, float hi, float& a, float& b, float& d) {
+template <class T>
+__device__ __forceinline__ void act_rule(int act, T lo, T hi, T& a, T& b, T& d) {
   switch (act) {
     case ACT_RELU: relu_rule(lo, hi, a, b, d); break;
     case ACT_ELU: elu_rule(lo, hi, a, b, d); break;
     case ACT_SIN: sin_rule(lo, hi, a, b, d); break;
     case ACT_TANH: tanh_rule(lo, hi, a, b, d); break;
-    default: a = 1.f; b = 0.f; d = 0.f; break;
+    default: a = T(1); b = T(0); d = T(0); break;
   }
 }
 
-__device__ __forceinline__ int sign_code(float lo, float hi, float offset) {
+template <class T>
+__device__ __forceinline__ int sign_code(T lo, T hi, T offset) {
   int c = CODE_UNKNOWN;
   if (lo > offset) c = CODE_POSITIVE;
   if (hi < -offset) c = CODE_NEGATIVE;
@@ -199,97 +248,98 @@ struct Planes {
 };
 
 // affine (interval / affine_fixed) hidden rule, src/affine.py:164-176
-template <int V>
-__device__ __forceinline__ void affine_fixed_rule(int act, float* v) {
+template <int V, class T>
+__device__ __forceinline__ void affine_fixed_rule(int act, T* v) {
   if (act == ACT_NONE) return;
-  float rad = 0.f;
+  T rad = T(0);
 #pragma unroll
-  for (int i = 0; i < V; ++i) rad = rad + fabsf(v[1 + i]);
+  for (int i = 0; i < V; ++i) rad = rad + Fm<T>::abs(v[1 + i]);
   rad = rad + v[V + 1];
-  float lo = v[0] - rad, hi = v[0] + rad;
-  float a, b, d;
+  T lo = v[0] - rad, hi = v[0] + rad;
+  T a, b, d;
   act_rule(act, lo, hi, a, b, d);
   v[0] = a * v[0] + b;
 #pragma unroll
   for (int i = 0; i < V; ++i) v[1 + i] = a * v[1 + i];
-  v[V + 1] = a * v[V + 1] + fabsf(d);
+  v[V + 1] = a * v[V + 1] + Fm<T>::abs(d);
 }
 
 // slope-interval hidden rule, src/slope_interval_layers.py:36-109
-template <int V>
-__device__ __forceinline__ void slope_rule(int act, float* v) {
+template <int V, class T>
+__device__ __forceinline__ void slope_rule(int act, T* v) {
   if (act == ACT_NONE) return;
-  float sl[V], su[V];
-  float rad = 0.f;
+  T sl[V], su[V];
+  T rad = T(0);
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     sl[i] = v[1 + i] - v[1 + V + i];
     su[i] = v[1 + i] + v[1 + V + i];
-    rad = rad + fmaxf(su[i], -sl[i]);
+    rad = rad + Fm<T>::max(su[i], -sl[i]);
   }
-  float pl = v[0] - rad, pu = v[0] + rad;
-  float dfl, dfu;
+  T pl = v[0] - rad, pu = v[0] + rad;
+  T dfl, dfu;
   bool general = false;
   if (act == ACT_RELU) {
-    dfl = pl > 0.f ? 1.f : 0.f;
-    dfu = pu < 0.f ? 0.f : 1.f;
+    dfl = pl > T(0) ? T(1) : T(0);
+    dfu = pu < T(0) ? T(0) : T(1);
   } else if (act == ACT_ELU) {
-    dfl = fminf(expf(pl), 1.f);
-    dfu = fminf(expf(pu), 1.f);
+    dfl = Fm<T>::min(Fm<T>::exp(pl), T(1));
+    dfu = Fm<T>::min(Fm<T>::exp(pu), T(1));
   } else if (act == ACT_TANH) {
-    float tl = tanhf(pl), tu = tanhf(pu);
-    float dl = 1.f - tl * tl, du = 1.f - tu * tu;
-    dfl = fminf(dl, du);
-    dfu = (pl <= 0.f && pu >= 0.f) ? 1.f : fmaxf(dl, du);
+    T tl = Fm<T>::tanh(pl), tu = Fm<T>::tanh(pu);
+    T dl = T(1) - tl * tl, du = T(1) - tu * tu;
+    dfl = Fm<T>::min(dl, du);
+    dfu = (pl <= T(0) && pu >= T(0)) ? T(1) : Fm<T>::max(dl, du);
   } else {  // sin: derivative cos may change sign -> full interval product
-    cos_bound_f(pl, pu, dfl, dfu);
+    cos_bound_t(pl, pu, dfl, dfu);
     general = true;
   }
 #pragma unroll
   for (int i = 0; i < V; ++i) {
-    float nl, nu;
+    T nl, nu;
     if (!general) {
-      nl = fminf(sl[i] * dfl, sl[i] * dfu);
-      nu = fmaxf(su[i] * dfl, su[i] * dfu);
+      nl = Fm<T>::min(sl[i] * dfl, sl[i] * dfu);
+      nu = Fm<T>::max(su[i] * dfl, su[i] * dfu);
     } else {
-      float p0 = sl[i] * dfl, p1 = sl[i] * dfu, p2 = su[i] * dfl, p3 = su[i] * dfu;
-      nl = fminf(fminf(fminf(p0, p1), p2), p3);
-      nu = fmaxf(fmaxf(fmaxf(p0, p1), p2), p3);
+      T p0 = sl[i] * dfl, p1 = sl[i] * dfu, p2 = su[i] * dfl, p3 = su[i] * dfu;
+      nl = Fm<T>::min(Fm<T>::min(Fm<T>::min(p0, p1), p2), p3);
+      nu = Fm<T>::max(Fm<T>::max(Fm<T>::max(p0, p1), p2), p3);
     }
-    float c = 0.5f * (nl + nu);
+    T c = T(0.5) * (nl + nu);
     v[1 + i] = c;
     v[1 + V + i] = nu - c;
   }
   v[0] = act_point(act, v[0]);
 }
 
-// Final bounds of a range form (output feature 0): lo, hi.
-template <int KIND, int V>
-__device__ __forceinline__ void final_bounds(const float* v, float& lo, float& hi, float& scale) {
+// Final bounds of a range form (output feature 0): lo, hi and the magnitude
+// scale |base| + radius.
+template <int KIND, int V, class T>
+__device__ __forceinline__ void final_bounds(const T* v, T& lo, T& hi, T& scale) {
   if constexpr (KIND == 0) {
-    float rad = 0.f;
+    T rad = T(0);
 #pragma unroll
-    for (int i = 0; i < V; ++i) rad = rad + fabsf(v[1 + i]);
+    for (int i = 0; i < V; ++i) rad = rad + Fm<T>::abs(v[1 + i]);
     rad = rad + v[V + 1];
     lo = v[0] - rad; hi = v[0] + rad;
-    scale = fabsf(v[0]) + rad;
+    scale = Fm<T>::abs(v[0]) + rad;
   } else if constexpr (KIND == 1) {
-    float rad = 0.f;
+    T rad = T(0);
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      float sl = v[1 + i] - v[1 + V + i];
-      float su = v[1 + i] + v[1 + V + i];
-      rad = rad + fmaxf(su, -sl);
+      T sl = v[1 + i] - v[1 + V + i];
+      T su = v[1 + i] + v[1 + V + i];
+      rad = rad + Fm<T>::max(su, -sl);
     }
     lo = v[0] - rad; hi = v[0] + rad;
-    scale = fabsf(v[0]) + rad;
+    scale = Fm<T>::abs(v[0]) + rad;
   } else {
-    lo = hi = v[0]; scale = fabsf(v[0]);
+    lo = hi = v[0]; scale = Fm<T>::abs(v[0]);
   }
 }
 
-template <class PL, int KIND, int V>
-__device__ __forceinline__ void apply_hidden(int act, float* v) {
+template <class PL, int KIND, int V, class T>
+__device__ __forceinline__ void apply_hidden(int act, T* v) {
   if constexpr (KIND == 0) affine_fixed_rule<V>(act, v);
   else if constexpr (KIND == 1) slope_rule<V>(act, v);
 #pragma unroll

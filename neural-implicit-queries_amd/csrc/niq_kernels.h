@@ -100,6 +100,75 @@ k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
   }
 }
 
+// ------------------------------------------------ FP64 near-threshold re-test
+// One workgroup per listed box: the same range-analysis forward in FP64 on the
+// vector ALUs, overwriting the FP32 sign code.  Used for the few cells whose
+// FP32 bound lies within recheck_tol * scale of the threshold, so that a cell
+// is only reported empty when the FP64 restatement agrees.
+template <int KIND, int V, int WMAX>
+__global__ void __launch_bounds__(256)
+k_recheck_fp64(const DevModel* __restrict__ Mp, ClassifyArgs a) {
+  using PL = Planes<KIND, V, 0>;
+  constexpr int R = PL::R;
+  __shared__ double X[2][R][WMAX];
+  const DevModel& M = *Mp;
+  const int n = min(*a.recheck_cnt, a.recheck_cap);
+  for (int j = blockIdx.x; j < n; j += gridDim.x) {
+    const long long i = a.recheck_idx[j];
+    for (int e = threadIdx.x; e < R * 16; e += blockDim.x) {
+      const int p = e / 16, k = e % 16;
+      double val = 0.0;
+      if (k < a.d) {
+        if (a.lower) {
+          const double lo = a.lower[i * a.d + k], up = a.upper[i * a.d + k];
+          const double c = 0.5 * (lo + up), h = up - c;
+          if (p == 0) val = c;
+          else if (KIND == 0 && V == 0) val = ::fabs(h);
+          else if (p >= 1 && p <= V) val = (p - 1 == k) ? h : 0.0;
+        } else {
+          if (p == 0) val = a.center[i * a.d + k];
+          else if (KIND == 0 && V == 0) {
+            double s = 0.0;
+            for (int r = 0; r < a.vin; ++r) s = s + ::fabs((double)a.vecs[(i * a.vin + r) * a.d + k]);
+            val = s;
+          } else if (p >= 1 && p <= V) val = a.vecs[(i * a.vin + (p - 1)) * a.d + k];
+        }
+      }
+      X[0][p][k] = val;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int l = 0; l < M.n_layers; ++l) {
+      const DevLayer& Ly = M.L[l];
+      const bool last = (l == M.n_layers - 1);
+      for (int f = threadIdx.x; f < Ly.out_pad; f += blockDim.x) {
+        double v[R];
+#pragma unroll
+        for (int p = 0; p < R; ++p) v[p] = 0.0;
+        for (int k = 0; k < Ly.in_pad; ++k) {
+          const double w = (double)Ly.W[(size_t)k * Ly.out_pad + f];
+#pragma unroll
+          for (int p = 0; p < R; ++p) v[p] = v[p] + X[cur][p][k] * (PL::absw(p) ? ::fabs(w) : w);
+        }
+#pragma unroll
+        for (int p = 0; p < R; ++p)
+          if (PL::bias(p)) v[p] = v[p] + (double)Ly.bias[f];
+        if (!last) {
+          apply_hidden<PL, KIND, V>(Ly.act, v);
+#pragma unroll
+          for (int p = 0; p < R; ++p) X[cur ^ 1][p][f] = v[p];
+        } else if (f == 0) {
+          double lo, hi, scale;
+          final_bounds<KIND, V>(v, lo, hi, scale);
+          a.codes[i] = sign_code<double>(lo, hi, (double)a.offset);
+        }
+      }
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+}
+
 // ------------------------------------------------------------ point values
 template <int NB, int NW, int WMAX>
 __global__ void __launch_bounds__(NW * 64)
@@ -130,7 +199,8 @@ struct TreeCounters {
   int n_int, n_ext;
   long long n_classified;
   int overflow;
-  int pad_;
+  int recheck_cnt;       // cells listed for the FP64 re-test in this level
+  long long n_recheck;   // over all levels
   int level_n[64];
   int level_int[64];
   int level_ext[64];
@@ -207,6 +277,8 @@ k_tree_scan(const int* __restrict__ bcnt, int* __restrict__ boff, TreeCounters* 
   if (threadIdx.x == 0) {
     const int U = carry[0];
     C->level_n[level] = n;
+    C->n_recheck += C->recheck_cnt;
+    C->recheck_cnt = 0;
     C->level_int[level] = C->n_int;
     C->level_ext[level] = C->n_ext;
     C->n_classified += n;

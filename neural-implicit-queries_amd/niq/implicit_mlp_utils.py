@@ -25,7 +25,7 @@ def implicit_from_params(params, mode, **kwargs):
     elif mode == 'affine_all':
         ctx = affine.AffineContext('affine_all')
     elif mode == 'slope_interval':
-        return slope_interval.SlopeIntervalImplicitFunction(None), params
+        return slope_interval.SlopeIntervalImplicitFunction(None, recheck_tol), params
     else:
         raise RuntimeError("unrecognized mode")
     return affine.AffineImplicitFunction(None, ctx, precision, recheck_tol), params

@@ -1,10 +1,19 @@
 """GPU parity: the gfx950 kernels (through the C ABI) against the oracle.
 
 Bars (written here, checked per case):
-  * plain values and bounds: |gpu - oracle32| <= 1e-5 * (|oracle| + mean |oracle|)
-    (MFMA and BLAS sum in different orders; nothing else differs);
+  * plain values: |gpu - oracle32| <= 1e-5 * max |oracle| over the batch
+    (relative to the output's magnitude: MFMA and BLAS sum in different
+    orders, nothing else differs);
+  * bounds: |gpu - oracle32| <= 1e-5 * max |oracle| + 2 |oracle32 - oracle64|
+    per cell.  On deep nets the affine bound of a small box amplifies rounding
+    (its interval part grows ~18x per 256-wide layer), so the FP32 oracle is
+    itself up to ~1e-4 * scale from FP64 there; the GPU must be no further from
+    the FP32 oracle than twice that;
   * sign codes: identical except cells whose FP64 bound lies within
-    1e-6 * scale of the threshold (scale = |lower| + |upper|), which are counted;
+    1e-6 * scale + 4 |oracle32 - oracle64| of the threshold (the cell's FP32
+    rounding noise), which are counted;
+  * with the FP64 re-test on (recheck_tol = 1e-3), no cell is reported
+    POSITIVE/NEGATIVE unless the FP64 oracle reports the same sign;
   * tree lists: identical arrays (order included) when no near-threshold
     decision differs;
   * rays: identical (t, hit_id, count) for all but near-threshold rays.
@@ -45,8 +54,7 @@ def boxes(n, seed, scale):
 
 
 def rel_close(got, ref, tol=1e-5):
-    scale = np.abs(ref) + np.mean(np.abs(ref))
-    return np.abs(got - ref) <= tol * scale
+    return np.abs(got - ref) <= tol * np.max(np.abs(ref))
 
 
 MODELS = ["c1_relu_3x32", "c2_relu_256x6", "c4_relu_512x8", "small_elu", "small_sin", "small_tanh"]
@@ -77,16 +85,26 @@ def test_classify_matches_oracle(name, mode):
     of32 = ofuncs.OracleFunc(p, mode, np.float32)
     of64 = ofuncs.OracleFunc(p, mode, np.float64)
     r_lo, r_hi = of32.bounds_box(lo, up)
-    assert np.all(rel_close(bounds[:, 0], r_lo)), np.max(np.abs(bounds[:, 0] - r_lo))
-    assert np.all(rel_close(bounds[:, 1], r_hi)), np.max(np.abs(bounds[:, 1] - r_hi))
-    ref_codes = oaff.sign_code(r_lo, r_hi, 0.0)
     d_lo, d_hi = of64.bounds_box(lo, up)
+    noise_lo, noise_hi = np.abs(r_lo - d_lo), np.abs(r_hi - d_hi)
+    e_lo, e_hi = np.abs(bounds[:, 0] - r_lo), np.abs(bounds[:, 1] - r_hi)
+    assert np.all(e_lo <= 1e-5 * np.max(np.abs(r_lo)) + 2 * noise_lo), np.max(e_lo)
+    assert np.all(e_hi <= 1e-5 * np.max(np.abs(r_hi)) + 2 * noise_hi), np.max(e_hi)
+    ref_codes = oaff.sign_code(r_lo, r_hi, 0.0)
     scale = np.abs(d_lo) + np.abs(d_hi)
-    near = (np.abs(d_lo) <= NEAR_REL * scale) | (np.abs(d_hi) <= NEAR_REL * scale)
+    near = (np.abs(d_lo) <= NEAR_REL * scale + 4 * noise_lo) | (np.abs(d_hi) <= NEAR_REL * scale + 4 * noise_hi)
     mism = codes != ref_codes
-    note(f"  {mism.sum()} code mismatches, {near.sum()} near-threshold cells")
+    note(f"  {mism.sum()} code mismatches, {near.sum()} near-threshold cells, "
+         f"max rel bound dev {np.max(np.maximum(e_lo, e_hi) / scale):.2e}")
     assert not np.any(mism & ~near)
     assert (codes != 0).sum() > 0 or mode == "interval"
+    # FP64 re-test: zero false-empty cells against the FP64 oracle
+    func64, _ = implicit_mlp_utils.implicit_from_params(p, mode, recheck_tol=1e-3)
+    c2 = func64.classify_box(p, torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda")).cpu().numpy()
+    c64 = oaff.sign_code(d_lo, d_hi, 0.0)
+    false_empty = (c2 != 0) & (c2 != c64)
+    note(f"  with FP64 re-test: {false_empty.sum()} false-empty, {(c2 != codes).sum()} codes changed")
+    assert not np.any(false_empty)
 
 
 def test_general_boxes_match_oracle():
@@ -142,10 +160,11 @@ def test_tree_c2_depth18_properties():
     """Full-size C2 tree: size-independent properties plus a sampled FP64 check."""
     note("[tree c2 depth 18]")
     p = model("c2_relu_256x6")
-    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", recheck_tol=1e-3)
     out, st = kd_tree.build_tree(func, p, LO, UP, split_depth=18, with_interior_nodes=True,
                                  with_exterior_nodes=True)
-    note(f"  {st['n_classified']} boxes bounded, {st['n_unknown']} unknown leaves")
+    note(f"  {st['n_classified']} boxes bounded, {st['n_unknown']} unknown leaves, "
+         f"{st['n_recheck']} FP64 re-tests")
     assert st["n_classified"] == sum(st["level_counts"])
     n = st["n_unknown"]
     lo = out["unknown_node_lower"][:n]
