@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--ray-res", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-precision", action="store_true", help="skip the BF16 vs FP32 comparison")
+    ap.add_argument("--precision-depth", type=int, default=21)
     return ap.parse_args()
 
 
@@ -63,6 +65,31 @@ def cpu_baseline(params, target_s):
                       f"({boxes} boxes, {secs:.1f} s, NumPy FP32 with {threads} BLAS threads)"}
 
 
+def precision_compare(depth):
+    """ReLU MLP 3->512x8->1 (He init, seed 2): the level-set tree to `depth`
+    with the plain FP32 contraction and with the BF16-MFMA contraction plus
+    rigorous widening.  One warm-up and one timed build each."""
+    import torch
+    from niq import models, implicit_mlp_utils, kd_tree
+    p = models.config_model("c4_relu_512x8")
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    res = {"workload": f"levelset tree, ReLU MLP 3->512x8->1, affine_fixed, split depth {depth}"}
+    dims = [3] + [512] * 8 + [1]
+    for prec in ("fp32", "bf16"):
+        f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision=prec)
+        kd_tree.build_tree(f, p, lo, up, split_depth=min(depth, 12))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, st = kd_tree.build_tree(f, p, lo, up, split_depth=depth)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[prec] = {"seconds": dt, "boxes": st["n_classified"], "boxes_per_s": st["n_classified"] / dt,
+                     "unknown_leaves": st["n_unknown"],
+                     "tflops": models.flops_per_box(dims, 5) * st["n_classified"] / dt / 1e12}
+    res["speedup_bf16_over_fp32"] = res["fp32"]["seconds"] / res["bf16"]["seconds"]
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -87,8 +114,11 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     n_boxes = 0
+    lat = []
     for _ in range(args.steps):
+        ts = time.perf_counter()
         _, st = step()
+        lat.append(time.perf_counter() - ts)
         n_boxes += st["n_classified"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -150,6 +180,9 @@ def main():
                      "kernel": "k_classify (affine_fixed, W=256)", "launch_ms": cls_ms,
                      "flops_per_box": flops_box, "boxes_per_launch": n_cls},
     }
+    line["latency_ms"] = {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)}
+    if not args.no_precision:
+        line["precision"] = precision_compare(args.precision_depth)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(params, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -1,9 +1,11 @@
 // C ABI of the gfx950 range-analysis hot path (declarations in include/niq.h).
 #include "niq.h"
 #include "niq_kernels.h"
+#include "niq_bf16.h"
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -27,6 +29,30 @@ niq_status fail(niq_status s, const std::string& msg) {
   } while (0)
 
 inline int round16(int x) { return (x + 15) / 16 * 16; }
+inline int round32(int x) { return (x + 31) / 32 * 32; }
+
+// host bf16 conversions (same bit rules as the device)
+inline unsigned short h_bf16_rn(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+inline float h_bf16_f(unsigned short h) {
+  unsigned u = (unsigned)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+inline unsigned short h_bf16_up_nonneg(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  unsigned short h = (unsigned short)(u >> 16);
+  if ((u & 0xffffu) && (u & 0x7fffffffu) < 0x7f800000u) h = (unsigned short)(h + 1);
+  return h;
+}
+// gamma_n(u) = n u / (1 - n u), rounded up generously
+inline double gamma_n(double n, double u) { return n * u / (1.0 - n * u) * (1.0 + 1e-6); }
 
 // Template width that holds every padded layer width of a model.
 inline int template_width(int wmax) {
@@ -76,6 +102,10 @@ struct niq_model {
   int n_layers = 0;
   int in_dim = 0;
   int wt = 0;                  // template width
+  int wt16 = 0;                // template width of the precision-mode copy
+  DevModelBf16 host16;         // precision-mode copy (bf16 hi/lo/|W| weights)
+  DevModelBf16* dev16 = nullptr;
+  void* weights16 = nullptr;
   std::vector<int> dims, acts;
   DevModel host;               // with device pointers
   DevModel* dev = nullptr;     // device copy of `host`
@@ -174,6 +204,85 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
     return fail(NIQ_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
   }
   m->bytes = total + sizeof(DevModel);
+
+  // Precision-mode copy: K padded to 32, hidden outputs padded to 32.
+  {
+    std::vector<size_t> o16(n_layers), ob16(n_layers);
+    size_t t16 = 0;
+    for (int i = 0; i < n_layers; ++i) {
+      const size_t ip = round32(dims[i]);
+      const size_t op = (i + 1 < n_layers) ? round32(dims[i + 1]) : round16(dims[i + 1]);
+      o16[i] = t16;
+      t16 += 3 * ip * op * sizeof(unsigned short);
+      t16 = (t16 + 255) / 256 * 256;
+      ob16[i] = t16;
+      t16 += op * sizeof(float);
+      t16 = (t16 + 255) / 256 * 256;
+    }
+    std::vector<unsigned char> h16(t16, 0);
+    int wmax16 = 32;
+    std::memset(&m->host16, 0, sizeof(DevModelBf16));
+    m->host16.n_layers = n_layers;
+    for (int i = 0; i < n_layers; ++i) {
+      const int in = dims[i], outd = dims[i + 1];
+      const int ip = round32(in);
+      const int op = (i + 1 < n_layers) ? round32(outd) : round16(outd);
+      wmax16 = std::max(wmax16, std::max(ip, op));
+      const int nkb = ip / 32;
+      const size_t nfrag = (size_t)(op / 16) * nkb * 64 * 8;
+      unsigned short* wh = reinterpret_cast<unsigned short*>(h16.data() + o16[i]);
+      unsigned short* wl = wh + nfrag;
+      unsigned short* wa = wl + nfrag;
+      for (int ft = 0; ft < op / 16; ++ft)
+        for (int kb = 0; kb < nkb; ++kb)
+          for (int l = 0; l < 64; ++l)
+            for (int sl = 0; sl < 8; ++sl) {
+              const int k = 32 * kb + 8 * (l / 16) + sl;
+              const int f = 16 * ft + (l % 16);
+              const float w = (k < in && f < outd) ? W[i][(size_t)k * outd + f] : 0.f;
+              const size_t idx = ((size_t)(ft * nkb + kb) * 64 + l) * 8 + sl;
+              const unsigned short hb = h_bf16_rn(w);
+              wh[idx] = hb;
+              wl[idx] = h_bf16_rn(w - h_bf16_f(hb));
+              wa[idx] = h_bf16_up_nonneg(std::fabs(w));
+            }
+      float* bp = reinterpret_cast<float*>(h16.data() + ob16[i]);
+      for (int f = 0; f < outd; ++f) bp[f] = (b && b[i]) ? b[i][f] : 0.f;
+      DevLayerBf16& L = m->host16.L[i];
+      L.in_pad = ip;
+      L.out_pad = op;
+      L.act = (i + 1 < n_layers) ? acts[i] : ACT_NONE;
+      // hi.Whi + lo.Whi + hi.Wlo drops terms <= 3.1 * 2^-16 |X||W|; FP32
+      // accumulation of 3K exact products adds gamma_{3K} (unit 2^-23, so any
+      // FP32 rounding mode of the accumulator is covered)
+      L.c_split = (float)((3.1 * std::ldexp(1.0, -16) + gamma_n(3.0 * ip, std::ldexp(1.0, -23))) * 1.01);
+      L.e_fix = (float)(1.0 + 3.0 * gamma_n((double)ip, std::ldexp(1.0, -23)));
+    }
+    m->host16.wmax = wmax16;
+    m->wt16 = template_width(wmax16);
+    e = hipMalloc(&m->weights16, t16);
+    if (e == hipSuccess) e = hipMemcpy(m->weights16, h16.data(), t16, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&m->dev16), sizeof(DevModelBf16));
+    if (e != hipSuccess) {
+      niq_model_destroy(m);
+      return fail(NIQ_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
+    }
+    for (int i = 0; i < n_layers; ++i) {
+      DevLayerBf16& L = m->host16.L[i];
+      const size_t nfrag = (size_t)(L.out_pad / 16) * (L.in_pad / 32) * 64;
+      unsigned char* wb = static_cast<unsigned char*>(m->weights16) + o16[i];
+      L.Wh = reinterpret_cast<const bf16x8*>(wb);
+      L.Wl = L.Wh + nfrag;
+      L.Wa = L.Wl + nfrag;
+      L.bias = reinterpret_cast<const float*>(static_cast<unsigned char*>(m->weights16) + ob16[i]);
+    }
+    e = hipMemcpy(m->dev16, &m->host16, sizeof(DevModelBf16), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      niq_model_destroy(m);
+      return fail(NIQ_ERR_HIP, std::string("model upload: ") + hipGetErrorString(e));
+    }
+    m->bytes += t16 + sizeof(DevModelBf16);
+  }
   *out = m;
   return NIQ_OK;
 }
@@ -182,6 +291,8 @@ void niq_model_destroy(niq_model* m) {
   if (!m) return;
   if (m->weights) (void)hipFree(m->weights);
   if (m->dev) (void)hipFree(m->dev);
+  if (m->weights16) (void)hipFree(m->weights16);
+  if (m->dev16) (void)hipFree(m->dev16);
   delete m;
 }
 
@@ -227,12 +338,52 @@ niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long 
   return fail(NIQ_ERR_UNSUPPORTED, "width");
 }
 
-// Route a classification by (mode, number of box vectors).
+template <int V, int W>
+niq_status launch_bf16_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
+  if constexpr ((size_t)PlanesBf16<V>::NPL * NB * W * 2 > (size_t)kLdsLimit) {
+    return fail(NIQ_ERR_UNSUPPORTED, "precision-mode range form does not fit in LDS at this width");
+  } else {
+    auto k = k_classify_bf16<V, NB, NW, W>;
+    const long long tiles = (max_n + NB - 1) / NB;
+    if (tiles == 0) return NIQ_OK;
+    const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+    hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev16, a);
+    HIPCHK(hipGetLastError());
+    return NIQ_OK;
+  }
+}
+
+template <int V>
+niq_status launch_bf16(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  switch (m->wt16) {
+    case 64: return launch_bf16_w<V, 64>(m, a, max_n, s);
+    case 128: return launch_bf16_w<V, 128>(m, a, max_n, s);
+    case 256: return launch_bf16_w<V, 256>(m, a, max_n, s);
+    case 512: return launch_bf16_w<V, 512>(m, a, max_n, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
+
+// Route a classification by (mode, precision, number of box vectors).
 niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec,
                              const ClassifyArgs& a, long long max_n, hipStream_t s) {
   const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
   const int prec = cfg ? cfg->precision : NIQ_PREC_FP32;
-  if (prec != NIQ_PREC_FP32) return fail(NIQ_ERR_UNSUPPORTED, "precision mode not available for this call");
+  if (prec == NIQ_PREC_BF16) {
+    ClassifyArgs b = a;
+    b.recheck_tol = 0.f;   // the precision mode is rigorous by construction
+    if (mode == NIQ_MODE_INTERVAL) return launch_bf16<0>(m, b, max_n, s);
+    if (mode == NIQ_MODE_AFFINE_FIXED) {
+      switch (nvec) {
+        case 1: return launch_bf16<1>(m, b, max_n, s);
+        case 2: return launch_bf16<2>(m, b, max_n, s);
+        case 3: return launch_bf16<3>(m, b, max_n, s);
+      }
+    }
+    return fail(NIQ_ERR_UNSUPPORTED, "the precision mode supports interval and affine_fixed with 1..3 box vectors");
+  }
+  if (prec != NIQ_PREC_FP32) return fail(NIQ_ERR_INVALID, "unknown precision");
   if (mode == NIQ_MODE_INTERVAL) return launch_classify<0, 0>(m, a, max_n, s);
   if (mode == NIQ_MODE_AFFINE_FIXED) {
     switch (nvec) {
