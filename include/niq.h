@@ -157,6 +157,32 @@ niq_status niq_cast_rays(const niq_model* const* models, int n_models, const niq
                          int32_t* out_count, void* workspace, size_t workspace_bytes,
                          int64_t* n_evals, void* stream);
 
+/* ---- closest point ---------------------------------------------------------
+ * Replaces closest_point / closest_point_iter (src/kd_tree.py:766-802, 659-763)
+ * with the same batch-stack order (B = batch_process_size items per round), so
+ * results match the reference's.  query_points (Q, 3); out_dist (Q,) and
+ * out_loc (Q, 3) float32 (distance to a spanning node's centre plus its
+ * diagonal, and that node's centre; queries that never meet a spanning node
+ * keep +inf and (-777, -777, -777)).  Synchronising (every 8 rounds). */
+niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* cfg, const float lower[3],
+                             const float upper[3], const float* query_points, int64_t Q, float eps,
+                             int batch_process_size, float* out_dist, float* out_loc,
+                             int64_t* n_processed, void* stream);
+
+/* ---- marching cubes ----------------------------------------------------------
+ * Replaces the extraction half of hierarchical_marching_cubes
+ * (src/kd_tree.py:338-399, src/extract_cell.py:314-421): for n_leaves leaf boxes
+ * (n, 3), a dense marching-cubes pass on the 2^s-per-side subgrid of each leaf.
+ * Writes up to tri_capacity triangles (T, 3, 3) float32 in (leaf, subcell,
+ * slot) order; *n_tris receives the full count (NIQ_ERR_CAPACITY if larger
+ * than tri_capacity).  Synchronising. */
+niq_status niq_mc_extract(const niq_model* m, const float* leaf_lower, const float* leaf_upper,
+                          int64_t n_leaves, int n_subcell_depth, float* tri_out, int64_t tri_capacity,
+                          int64_t* n_tris, void* stream);
+/* The marching-cubes triangle table the kernels use: out[256 * 16] edge ids,
+ * -1 padded (host only; for checking). */
+int niq_mc_table(signed char* out);
+
 /* ---- diagnostics -------------------------------------------------------- */
 const char* niq_last_error(void);
 int niq_version(void);

@@ -1,18 +1,24 @@
 // Precision mode: the error-term contraction in BF16 on MFMA with FP32
 // accumulation, made rigorous again by widening.
 //
-// Each activation X (FP32) is held as X = hi + lo with hi = bf16(X),
-// lo = bf16(X - hi) (|X - hi - lo| <= 2^-16 |X|); weights likewise.  A
-// dense layer computes, per plane,  hi.Whi + lo.Whi + hi.Wlo  with
-// v_mfma_f32_16x16x32_bf16 (every bf16 x bf16 product is exact in FP32).  The
-// dropped terms and the FP32 accumulation are bounded a priori by
-//     |X.W - computed| <= c |X|.|W|,   c = 3.1 * 2^-16 + gamma_{3K}(2^-23),
-// and summed over the planes of a box this is c (sum_p |X_p|).|W|.  That
-// bound rides in one extra non-negative plane e = err + c sum_p |X_p|,
-// rounded up to bf16 and multiplied by |W| rounded up to bf16, so the
-// computed interval error already contains the widening.  Elementwise FP32
-// steps carry a few-ulp margin, and the last radius-versus-threshold test is
-// done in FP64.  Only the affine forms (interval, affine_fixed) run here.
+// Activations and weights are split into bf16 terms: the base row (magnitude
+// O(1)) three ways, X = x1 + x2 + x3 (|residual| <= 2^-24 |X|, W likewise),
+// the error-term rows (magnitude ~ box size) two ways.  A dense layer then
+// sums exact bf16 x bf16 products with v_mfma_f32_16x16x32_bf16, smallest
+// terms first:
+//     base:  x3 w1 + x2 w2 + x1 w3 + x2 w1 + x1 w2 + x1 w1
+//     aff:   a2 w1 + a1 w2 + a1 w1
+// into a fresh accumulator per 32-wide K chunk, and the chunk results are
+// added in FP32.  The dropped terms plus both rounding stages are bounded a
+// priori by |X.W - computed| <= c |X|.|W|, with
+//     c = split residual + 1.05 gamma_33(2^-23) + gamma_{K/32}(2^-24)
+// (unit 2^-23 covers any FP32 rounding mode inside the MFMA), so c_base is a
+// few 1e-6 and c_aff a few 1e-5.  The bound rides in one extra non-negative
+// plane e = err + c_base |base| + c_aff sum_i |aff_i|, rounded up to bf16 and
+// multiplied by |W| rounded up to bf16: the computed interval error already
+// contains the widening.  Elementwise FP32 steps carry a few-ulp margin, and
+// the last radius-versus-threshold test is done in FP64.  Only the affine
+// forms (interval, affine_fixed) run here.
 #pragma once
 #include "niq_device.h"
 
@@ -26,12 +32,15 @@ struct DevLayerBf16 {
   int out_pad;   // multiple of 16
   int act;
   int pad_;
-  const bf16x8* Wh;   // bf16(W)
-  const bf16x8* Wl;   // bf16(W - bf16(W))
+  const bf16x8* W1;   // w1 = bf16(W)
+  const bf16x8* W2;   // w2 = bf16(W - w1)
+  const bf16x8* W3;   // w3 = bf16(W - w1 - w2)
   const bf16x8* Wa;   // |W| rounded up to bf16
   const float* bias;
-  float c_split;      // a-priori contraction bound c of this layer
-  float e_fix;        // (1 + 3 gamma_K): FP32 accumulation of the e plane
+  float c_base;       // a-priori contraction bound of the base row
+  float c_aff;        // ... of the error-term rows
+  float e_fix;        // 1 + bound of the (non-negative) e-plane accumulation
+  float pad2_;
 };
 
 struct DevModelBf16 {
@@ -68,29 +77,38 @@ __device__ __forceinline__ int boff16(int plane, int b, int k) {
   return (plane * NB + b) * WMAX + ((((k >> 3) ^ (b & 7)) << 3) | (k & 7));
 }
 
-// Planes: for p in 0..V: hi at plane 2p, lo at plane 2p+1 (base p = 0,
-// aff_i p = 1 + i); the e plane is plane 2(V+1).
+// LDS planes: base x1, x2, x3 at 0, 1, 2; aff_i terms a1, a2 at 3 + 2i,
+// 4 + 2i; the e plane last.
 template <int V> struct PlanesBf16 {
-  static constexpr int NS = V + 1;          // split planes
-  static constexpr int NPL = 2 * NS + 1;    // bf16 planes in LDS
+  static constexpr int NS = V + 1;          // affine rows (base + aff)
+  static constexpr int NPL = 3 + 2 * V + 1; // bf16 planes in LDS
+  static constexpr int E = 3 + 2 * V;       // index of the e plane
 };
 
 constexpr float kU24 = 5.9604644775390625e-08f;   // 2^-24
 
-// Store one (box, feature) element's post-activation affine form.
+// Split and store one (box, feature) element's affine form: base three ways,
+// aff two ways, and the e plane err + c_base |base| + c_aff sum |aff|.
 template <int V, int NB, int WMAX>
-__device__ __forceinline__ void store_split(unsigned short* Xs, int b, int k, const float* s, float err, float c) {
-  float mag = 0.f;
+__device__ __forceinline__ void store_split(unsigned short* Xs, int b, int k, const float* s, float err,
+                                            float c_base, float c_aff) {
+  const unsigned short x1 = bf16_bits_rn(s[0]);
+  const float r1 = s[0] - bf16_to_f(x1);
+  const unsigned short x2 = bf16_bits_rn(r1);
+  const unsigned short x3 = bf16_bits_rn(r1 - bf16_to_f(x2));
+  Xs[boff16<NB, WMAX>(0, b, k)] = x1;
+  Xs[boff16<NB, WMAX>(1, b, k)] = x2;
+  Xs[boff16<NB, WMAX>(2, b, k)] = x3;
+  float amag = 0.f;
 #pragma unroll
-  for (int p = 0; p <= V; ++p) {
-    const unsigned short h = bf16_bits_rn(s[p]);
-    const unsigned short l = bf16_bits_rn(s[p] - bf16_to_f(h));
-    Xs[boff16<NB, WMAX>(2 * p, b, k)] = h;
-    Xs[boff16<NB, WMAX>(2 * p + 1, b, k)] = l;
-    mag = mag + fabsf(s[p]);
+  for (int i = 0; i < V; ++i) {
+    const unsigned short h = bf16_bits_rn(s[1 + i]);
+    Xs[boff16<NB, WMAX>(3 + 2 * i, b, k)] = h;
+    Xs[boff16<NB, WMAX>(4 + 2 * i, b, k)] = bf16_bits_rn(s[1 + i] - bf16_to_f(h));
+    amag = amag + fabsf(s[1 + i]);
   }
-  const float e = up1(up1(err + c * mag));
-  Xs[boff16<NB, WMAX>(2 * (V + 1), b, k)] = bf16_bits_up_nonneg(e);
+  const float e = up1(up1(err + c_base * fabsf(s[0]) + up1(c_aff * up1(amag * (1.f + 4.f * kU24)))));
+  Xs[boff16<NB, WMAX>(3 + 2 * V, b, k)] = bf16_bits_up_nonneg(e);
 }
 
 // Rigorous affine rule for one element: v[0] base, v[1..V] aff, v[V+1] err
@@ -139,14 +157,16 @@ __device__ __forceinline__ void affine_rule_rigorous(int act, float* v) {
 // err rigorous) of output feature 0.
 template <int V, int NB, int NW, int WMAX, class Fin>
 __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned short* __restrict__ Xs, Fin&& fin) {
+  using PB = PlanesBf16<V>;
   constexpr int NS = V + 1;
-  constexpr int R = NS + 1;   // accumulators: split planes + e plane
+  constexpr int R = NS + 1;   // accumulators: base, aff_i, e
   constexpr int BT = NB / 16;
   constexpr int FTMAX = (WMAX / 16 + NW - 1) / NW;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int ln = lane & 15;
   const int lg = lane >> 4;
+  const floatx4 z4 = {0.f, 0.f, 0.f, 0.f};
 
   for (int l = 0; l < M.n_layers; ++l) {
     const DevLayerBf16 Ly = M.L[l];
@@ -159,48 +179,53 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
 #pragma unroll
       for (int p = 0; p < R; ++p)
 #pragma unroll
-        for (int t = 0; t < BT; ++t) acc[f][p][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < BT; ++t) acc[f][p][t] = z4;
 
     __syncthreads();
     for (int kb = 0; kb < nkb; ++kb) {
-      bf16x8 wh[FTMAX], wl[FTMAX], wa[FTMAX];
-#pragma unroll
-      for (int f = 0; f < FTMAX; ++f) {
-        const int ft = wave + f * NW;
-        if (ft < nft) {
-          const size_t o = (size_t)(ft * nkb + kb) * 64 + lane;
-          wh[f] = Ly.Wh[o]; wl[f] = Ly.Wl[o]; wa[f] = Ly.Wa[o];
-        }
-      }
 #pragma unroll
       for (int t = 0; t < BT; ++t) {
         const int b = t * 16 + ln;
         const int k0 = kb * 32 + lg * 8;
-#pragma unroll
-        for (int p = 0; p < NS; ++p) {
-          const bf16x8 xh = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(2 * p, b, k0)]);
-          const bf16x8 xl = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(2 * p + 1, b, k0)]);
-#pragma unroll
-          for (int f = 0; f < FTMAX; ++f) {
-            const int ft = wave + f * NW;
-            if (ft < nft) {
-              acc[f][p][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[f], xh, acc[f][p][t], 0, 0, 0);
-              acc[f][p][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[f], xl, acc[f][p][t], 0, 0, 0);
-              acc[f][p][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[f], xh, acc[f][p][t], 0, 0, 0);
-            }
-          }
-        }
-        const bf16x8 xe = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(2 * NS, b, k0)]);
+        const bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(0, b, k0)]);
+        const bf16x8 x2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(1, b, k0)]);
+        const bf16x8 x3 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(2, b, k0)]);
+        const bf16x8 xe = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(PB::E, b, k0)]);
 #pragma unroll
         for (int f = 0; f < FTMAX; ++f) {
           const int ft = wave + f * NW;
-          if (ft < nft) acc[f][NS][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[f], xe, acc[f][NS][t], 0, 0, 0);
+          if (ft >= nft) continue;
+          const size_t o = (size_t)(ft * nkb + kb) * 64 + lane;
+          const bf16x8 w1 = Ly.W1[o], w2 = Ly.W2[o];
+          // base row, smallest products first, fresh accumulator per chunk
+          {
+            const bf16x8 w3 = Ly.W3[o];
+            floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x3, z4, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, x2, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, x1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x2, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, x1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x1, c, 0, 0, 0);
+            acc[f][0][t] += c;
+          }
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k0)]);
+            const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k0)]);
+            floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a2, z4, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, a1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a1, c, 0, 0, 0);
+            acc[f][1 + i][t] += c;
+          }
+          acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ly.Wa[o], xe, z4, 0, 0, 0);
         }
       }
     }
 
     __syncthreads();
 
+    const float cb_next = last ? 0.f : M.L[l + 1].c_base;
+    const float ca_next = last ? 0.f : M.L[l + 1].c_aff;
 #pragma unroll
     for (int f = 0; f < FTMAX; ++f) {
       const int ft = wave + f * NW;
@@ -209,7 +234,7 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
 #pragma unroll
       for (int t = 0; t < BT; ++t) {
         const int b = t * 16 + ln;
-        unsigned short hs[NS][4], ls[NS][4], es[4];
+        unsigned short xs[3][4], as[V > 0 ? V : 1][2][4], es[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           float v[V + 2];
@@ -222,28 +247,35 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
             continue;
           }
           affine_rule_rigorous<V>(Ly.act, v);
-          float mag = 0.f;
+          const unsigned short h1 = bf16_bits_rn(v[0]);
+          const float r1 = v[0] - bf16_to_f(h1);
+          const unsigned short h2 = bf16_bits_rn(r1);
+          xs[0][j] = h1; xs[1][j] = h2; xs[2][j] = bf16_bits_rn(r1 - bf16_to_f(h2));
+          float amag = 0.f;
 #pragma unroll
-          for (int p = 0; p < NS; ++p) {
-            hs[p][j] = bf16_bits_rn(v[p]);
-            ls[p][j] = bf16_bits_rn(v[p] - bf16_to_f(hs[p][j]));
-            mag = mag + fabsf(v[p]);
+          for (int i = 0; i < V; ++i) {
+            const unsigned short h = bf16_bits_rn(v[1 + i]);
+            as[i][0][j] = h;
+            as[i][1][j] = bf16_bits_rn(v[1 + i] - bf16_to_f(h));
+            amag = amag + fabsf(v[1 + i]);
           }
-          // next layer's widening rides in the e plane (c of the next layer)
-          const float cn = M.L[l + 1].c_split;
-          es[j] = bf16_bits_up_nonneg(up1(up1(v[V + 1] + cn * up1(mag * (1.f + 8.f * kU24)))));
+          // next layer's contraction bound rides in the e plane
+          es[j] = bf16_bits_up_nonneg(up1(up1(v[V + 1] + cb_next * fabsf(v[0]) +
+                                              up1(ca_next * up1(amag * (1.f + 4.f * kU24))))));
         }
         if (!last) {
           const int k = ft * 16 + lg * 4;
 #pragma unroll
-          for (int p = 0; p < NS; ++p) {
-            ushort4 h4 = {hs[p][0], hs[p][1], hs[p][2], hs[p][3]};
-            ushort4 l4 = {ls[p][0], ls[p][1], ls[p][2], ls[p][3]};
-            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(2 * p, b, k)]) = h4;
-            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(2 * p + 1, b, k)]) = l4;
+          for (int q = 0; q < 3; ++q)
+            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(q, b, k)]) = ushort4{xs[q][0], xs[q][1], xs[q][2], xs[q][3]};
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k)]) =
+                ushort4{as[i][0][0], as[i][0][1], as[i][0][2], as[i][0][3]};
+            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k)]) =
+                ushort4{as[i][1][0], as[i][1][1], as[i][1][2], as[i][1][3]};
           }
-          ushort4 e4 = {es[0], es[1], es[2], es[3]};
-          *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(2 * NS, b, k)]) = e4;
+          *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(PB::E, b, k)]) = ushort4{es[0], es[1], es[2], es[3]};
         }
       }
     }
@@ -271,7 +303,7 @@ k_classify_bf16(const DevModelBf16* __restrict__ Mp, ClassifyArgs a) {
   __shared__ __attribute__((aligned(16))) unsigned short Xs[PB::NPL * NB * WMAX];
   const long long n = a.n_dev ? (long long)(*a.n_dev) : a.n;
   const long long ntiles = (n + NB - 1) / NB;
-  const float c0 = Mp->L[0].c_split;
+  const float cb0 = Mp->L[0].c_base, ca0 = Mp->L[0].c_aff;
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const long long i0 = tile * NB;
     // layer-0 inputs: features 0..31 (K padded to 32)
@@ -300,7 +332,7 @@ k_classify_bf16(const DevModelBf16* __restrict__ Mp, ClassifyArgs a) {
           for (int p = 1; p <= V; ++p) s[p] = a.vecs[(i * a.vin + (p - 1)) * a.d + k];
         }
       }
-      store_split<V, NB, WMAX>(Xs, b, k, s, err, c0);
+      store_split<V, NB, WMAX>(Xs, b, k, s, err, cb0, ca0);
     }
     mlp_forward_bf16<V, NB, NW, WMAX>(*Mp, Xs, [&](const float* v, int b) {
       const long long i = i0 + b;

@@ -192,6 +192,30 @@ k_points(const DevModel* __restrict__ Mp, const float* __restrict__ x, int d, lo
   }
 }
 
+// Point values with a device-resident count: n = (*n_dev) * mult points (3D).
+template <int NB, int NW, int WMAX>
+__global__ void __launch_bounds__(NW * 64)
+k_points_dev(const DevModel* __restrict__ Mp, const float* __restrict__ x, const int* __restrict__ n_dev, int mult,
+             float* __restrict__ out) {
+  using PL = Planes<2, 0, 1>;
+  constexpr int NT = NW * 64;
+  __shared__ float X[NB * WMAX];
+  const long long n = (long long)(*n_dev) * mult;
+  const long long ntiles = (n + NB - 1) / NB;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long i0 = tile * NB;
+    for (int e = threadIdx.x; e < NB * 16; e += NT) {
+      const int b = e >> 4, k = e & 15;
+      const long long i = i0 + b;
+      X[xoff<NB, WMAX>(0, b, k)] = (i < n && k < 3) ? x[i * 3 + k] : 0.f;
+    }
+    mlp_forward<PL, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+      const long long i = i0 + b;
+      if (i < n) out[i] = v[0];
+    });
+  }
+}
+
 // -------------------------------------------------- level-set tree step
 // Device counters of one tree build.  n[l & 1] is the node count of level l.
 struct TreeCounters {

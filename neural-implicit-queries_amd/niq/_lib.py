@@ -26,7 +26,8 @@ PREC = {"fp32": 0, "bf16": 1}
 EXPORTS = ("niq_model_create", "niq_model_destroy", "niq_model_device_bytes",
            "niq_classify_boxes", "niq_classify_general_boxes", "niq_eval_points",
            "niq_tree_workspace_bytes", "niq_tree_build", "niq_cast_workspace_bytes",
-           "niq_cast_rays", "niq_last_error", "niq_version")
+           "niq_cast_rays", "niq_closest_point", "niq_mc_extract", "niq_mc_table",
+           "niq_last_error", "niq_version")
 
 
 class NiqError(RuntimeError):
@@ -96,6 +97,9 @@ def load_library():
         "niq_tree_build": (I, [P, P, P, P, S, P, P]),
         "niq_cast_workspace_bytes": (S, [P]),
         "niq_cast_rays": (I, [P, I, P, P, P, I64, P, P, P, P, P, S, ctypes.POINTER(I64), P]),
+        "niq_closest_point": (I, [P, P, P, P, P, I64, F, I, P, P, ctypes.POINTER(I64), P]),
+        "niq_mc_extract": (I, [P, P, P, I64, I, P, I64, ctypes.POINTER(I64), P]),
+        "niq_mc_table": (I, [P]),
         "niq_last_error": (ctypes.c_char_p, []),
         "niq_version": (I, []),
     }

@@ -130,3 +130,57 @@ def construct_uniform_unknown_levelset_tree(func, params, lower, upper, node_ter
     out, _ = build_tree(func, params, lower, upper, node_terminate_thresh, split_depth,
                         with_interior_nodes, with_exterior_nodes, offset, batch_process_size)
     return out
+
+
+def closest_point(func, params, lower, upper, query_points, eps=0.001, batch_process_size=2048):
+    """src/kd_tree.py:766-802: (query_min_dist (Q,), query_min_loc (Q, 3)).
+
+    Runs the reference's batch-stack search on the device (niq_closest_point);
+    the round order is the reference's, so results match it."""
+    _lib.require_gpu()
+    lib = _lib.load_library()
+    m = _mlp.device_model(params)
+    q = torch.as_tensor(query_points).to(device="cuda", dtype=torch.float32).contiguous()
+    Q = q.shape[0]
+    lo = (ctypes.c_float * 3)(*[float(v) for v in torch.as_tensor(lower).reshape(-1).tolist()])
+    up = (ctypes.c_float * 3)(*[float(v) for v in torch.as_tensor(upper).reshape(-1).tolist()])
+    dist = torch.empty(Q, dtype=torch.float32, device=q.device)
+    loc = torch.empty((Q, 3), dtype=torch.float32, device=q.device)
+    n_proc = ctypes.c_int64(0)
+    _lib.check(lib.niq_closest_point(m.handle, ctypes.byref(func.cfg), lo, up, _lib.ptr(q), Q, float(eps),
+                                     int(batch_process_size), _lib.ptr(dist), _lib.ptr(loc),
+                                     ctypes.byref(n_proc), _lib.stream_handle()))
+    closest_point.last_n_processed = int(n_proc.value)
+    return dist, loc
+
+
+def extract_cells(params, node_lower, node_upper, n_leaves, n_subcell_depth):
+    """Marching cubes over the first n_leaves boxes: (T, 3, 3) triangles."""
+    lib = _lib.load_library()
+    m = _mlp.device_model(params)
+    lo = node_lower[:n_leaves].contiguous()
+    up = node_upper[:n_leaves].contiguous()
+    cap = max(1024, n_leaves * 8)
+    while True:
+        tri = torch.empty((cap, 3, 3), dtype=torch.float32, device=lo.device)
+        n = ctypes.c_int64(0)
+        st = lib.niq_mc_extract(m.handle, _lib.ptr(lo), _lib.ptr(up), n_leaves, int(n_subcell_depth),
+                                _lib.ptr(tri), cap, ctypes.byref(n), _lib.stream_handle())
+        if st == _lib.NIQ_ERR_CAPACITY:
+            cap = int(n.value)
+            continue
+        _lib.check(st)
+        return tri[:int(n.value)]
+
+
+def hierarchical_marching_cubes(func, params, lower, upper, depth, n_subcell_depth=2,
+                                extract_batch_max_tri_out=1000000):
+    """src/kd_tree.py:357-399: tree to split depth 3 (depth - n_subcell_depth),
+    then dense marching cubes on the 2^s subgrid of every unknown leaf.
+    ``extract_batch_max_tri_out`` only sized the reference's batches; the
+    device extraction chunks by memory instead."""
+    out, st = build_tree(func, params, lower, upper, split_depth=3 * (depth - n_subcell_depth))
+    n = st["n_unknown"]
+    if n == 0:
+        return torch.zeros((0, 3, 3), dtype=torch.float32, device="cuda")
+    return extract_cells(params, out['unknown_node_lower'], out['unknown_node_upper'], n, n_subcell_depth)

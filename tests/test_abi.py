@@ -62,3 +62,16 @@ def test_models_are_seeded_and_median_shifted():
         np.testing.assert_array_equal(a[k], b[k])
     v = omlp.eval_points(a, models.median_grid(), np.float64)
     assert abs(np.median(v)) < 1e-6
+
+
+def test_kernel_mc_table_matches_oracle_table():
+    """The C library's marching-cubes table equals the oracle's (both are
+    Bourke's public table in independent encodings)."""
+    import ctypes
+    from niq import _lib
+    from oracle import mc as omc
+    lib = _lib.load_library()
+    buf = (ctypes.c_byte * (256 * 16))()
+    assert lib.niq_mc_table(buf) == 0
+    got = np.frombuffer(bytes(buf), dtype=np.int8).reshape(256, 16)[:, :15].reshape(256, 5, 3)
+    np.testing.assert_array_equal(got, omc.tri_table())

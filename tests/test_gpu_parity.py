@@ -301,3 +301,67 @@ def test_bf16_tree_matches_fp32_up_to_inflation():
         u = b["interior_node_upper"][:m].cpu().numpy()[idx]
         assert np.all(ofuncs.OracleFunc(p, "affine_fixed", np.float64).classify_box(l, u) == 2)
     assert sb["n_unknown"] >= sa["n_unknown"] * 0.999
+
+
+# ------------------------------------------------------------ closest point
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "fox"])
+def test_closest_point_matches_oracle(name):
+    from oracle import closest as ocp
+    note(f"[closest point] {name}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    q = np.random.default_rng(3).uniform(-1, 1, size=(300, 3)).astype(np.float32)
+    eps = 0.01
+    d, loc = kd_tree.closest_point(func, p, LO, UP, torch.as_tensor(q, device="cuda"), eps=eps,
+                                   batch_process_size=256)
+    rd, rloc, info = ocp.closest_point(ofuncs.OracleFunc(p, "affine_fixed"), LO, UP, q, eps=eps,
+                                       batch_process_size=256)
+    d, loc = d.cpu().numpy(), loc.cpu().numpy()
+    same = (d == rd)
+    note(f"  {same.mean() * 100:.2f}% distances identical; gpu processed "
+         f"{kd_tree.closest_point.last_n_processed} nodes, oracle {info['n_processed']}")
+    assert same.mean() >= 0.98
+    fin = np.isfinite(rd) & same
+    np.testing.assert_array_equal(loc[fin], rloc[fin])
+
+
+def test_closest_point_plane_closed_form():
+    nrm = np.array([0., 0., 1.], np.float32)
+    A1 = np.concatenate([np.eye(3), -np.eye(3)], 1)
+    A2 = np.concatenate([0.5 * nrm, -0.5 * nrm])[:, None]
+    p = nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-0.1]),
+                         nmlp.squeeze_last()])
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    q = np.random.default_rng(0).uniform(-1, 1, size=(2000, 3)).astype(np.float32)
+    eps = 0.02
+    d, loc = kd_tree.closest_point(func, p, LO, UP, q, eps=eps)
+    true = np.abs(q[:, 2] - 0.1)
+    d = d.cpu().numpy()
+    assert np.all(d >= true - 1e-6) and np.all(d <= true + 4 * eps)
+
+
+# ------------------------------------------------------------ marching cubes
+@pytest.mark.parametrize("name,s", [("c1_relu_3x32", 2), ("fox", 3)])
+def test_marching_cubes_matches_oracle(name, s):
+    from oracle import mc as omc
+    note(f"[mc] {name} s={s}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    depth = 5 if s == 2 else 6
+    tri = kd_tree.hierarchical_marching_cubes(func, p, LO, UP, depth, n_subcell_depth=s).cpu().numpy()
+    ref, _ = omc.hierarchical_marching_cubes(ofuncs.OracleFunc(p, "affine_fixed"), LO, UP, depth, s)
+    note(f"  gpu {tri.shape[0]} triangles, oracle {ref.shape[0]}")
+    assert tri.shape == ref.shape
+    np.testing.assert_allclose(tri, ref, atol=2e-5)
+
+
+def test_marching_cubes_plane_area():
+    nrm = np.array([0., 0., 1.], np.float32)
+    A1 = np.concatenate([np.eye(3), -np.eye(3)], 1)
+    A2 = np.concatenate([0.5 * nrm, -0.5 * nrm])[:, None]
+    p = nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-0.3]),
+                         nmlp.squeeze_last()])
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    tri = kd_tree.hierarchical_marching_cubes(func, p, LO, UP, 6, n_subcell_depth=3).cpu().numpy()
+    a = np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=-1) / 2
+    assert abs(a.sum() - 4.0) < 1e-3
