@@ -3,6 +3,7 @@
 #include "niq_kernels.h"
 #include "niq_bf16.h"
 #include "niq_cp_mc.h"
+#include "niq_rows.h"
 
 #include <algorithm>
 #include <climits>
@@ -376,6 +377,41 @@ niq_status launch_bf16(const niq_model* m, const ClassifyArgs& a, long long max_
   return fail(NIQ_ERR_UNSUPPORTED, "width");
 }
 
+// affine_all: one box per workgroup, error terms streamed through HBM scratch.
+template <int W>
+niq_status launch_all_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  constexpr int NW = RowsCfg<W>::NW;
+  auto k = k_classify_all<W>;
+  long long max_rows = std::max(a.d, a.lower ? 0 : a.vin);
+  for (int i = 0; i + 1 < m->n_layers; ++i)
+    if (m->acts[i] != NIQ_ACT_NONE) max_rows += m->host.L[i].out_pad;
+  const size_t per_wg = sizeof(float) * 2 * (size_t)max_rows * W;
+  long long g = std::min<long long>(max_n, persistent_blocks(k, NW * 64));
+  g = std::max<long long>(1, std::min<long long>(g, (long long)((16ull << 30) / per_wg)));
+  if (max_n == 0) return NIQ_OK;
+  void* scratch = nullptr;
+  HIPCHK(hipMallocAsync(&scratch, per_wg * g, s));
+  RowsArgs ra{};
+  ra.c = a;
+  ra.scratch = static_cast<float*>(scratch);
+  ra.max_rows = max_rows;
+  hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(NW * 64), 0, s, m->dev, ra);
+  const hipError_t e = hipGetLastError();
+  HIPCHK(hipFreeAsync(scratch, s));
+  if (e != hipSuccess) return fail(NIQ_ERR_HIP, std::string("k_classify_all: ") + hipGetErrorString(e));
+  return NIQ_OK;
+}
+
+niq_status launch_all(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  switch (m->wt) {
+    case 64: return launch_all_w<64>(m, a, max_n, s);
+    case 128: return launch_all_w<128>(m, a, max_n, s);
+    case 256: return launch_all_w<256>(m, a, max_n, s);
+    case 512: return launch_all_w<512>(m, a, max_n, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
+
 // Route a classification by (mode, precision, number of box vectors).
 niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec,
                              const ClassifyArgs& a, long long max_n, hipStream_t s) {
@@ -395,6 +431,11 @@ niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nv
     return fail(NIQ_ERR_UNSUPPORTED, "the precision mode supports interval and affine_fixed with 1..3 box vectors");
   }
   if (prec != NIQ_PREC_FP32) return fail(NIQ_ERR_INVALID, "unknown precision");
+  if (mode == NIQ_MODE_AFFINE_ALL) {
+    if (a.recheck_tol > 0.f) return fail(NIQ_ERR_UNSUPPORTED, "affine_all has no FP64 re-test");
+    if (!a.lower && a.vin > 64) return fail(NIQ_ERR_UNSUPPORTED, "affine_all takes at most 64 box vectors");
+    return launch_all(m, a, max_n, s);
+  }
   if (mode == NIQ_MODE_INTERVAL) return launch_classify<0, 0>(m, a, max_n, s);
   if (mode == NIQ_MODE_AFFINE_FIXED) {
     switch (nvec) {

@@ -365,3 +365,56 @@ def test_marching_cubes_plane_area():
     tri = kd_tree.hierarchical_marching_cubes(func, p, LO, UP, 6, n_subcell_depth=3).cpu().numpy()
     a = np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=-1) / 2
     assert abs(a.sum() - 4.0) < 1e-3
+
+
+# ------------------------------------------------------------------ affine_all
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "small_tanh", "fox", "c3_tanh_128x8"])
+def test_affine_all_matches_oracle(name):
+    note(f"[affine_all] {name}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_all")
+    n = 300 if name == "c3_tanh_128x8" else 800
+    lo, up = boxes(n, seed=5, scale=0.2)
+    codes, bounds = func.classify_box(p, torch.as_tensor(lo, device="cuda"),
+                                      torch.as_tensor(up, device="cuda"), return_bounds=True)
+    codes, bounds = codes.cpu().numpy(), bounds.cpu().numpy()
+    r_lo, r_hi = ofuncs.OracleFunc(p, "affine_all", np.float32).bounds_box(lo, up)
+    d_lo, d_hi = ofuncs.OracleFunc(p, "affine_all", np.float64).bounds_box(lo, up)
+    noise_lo, noise_hi = np.abs(r_lo - d_lo), np.abs(r_hi - d_hi)
+    e_lo, e_hi = np.abs(bounds[:, 0] - r_lo), np.abs(bounds[:, 1] - r_hi)
+    scale = np.abs(d_lo) + np.abs(d_hi)
+    note(f"  max rel bound dev {np.max(np.maximum(e_lo, e_hi) / scale):.2e}, determinate {np.mean(codes != 0):.3f}")
+    assert np.all(e_lo <= 1e-5 * np.max(np.abs(r_lo)) + 2 * noise_lo)
+    assert np.all(e_hi <= 1e-5 * np.max(np.abs(r_hi)) + 2 * noise_hi)
+    ref_codes = oaff.sign_code(r_lo, r_hi, 0.0)
+    near = (np.abs(d_lo) <= NEAR_REL * scale + 4 * noise_lo) | (np.abs(d_hi) <= NEAR_REL * scale + 4 * noise_hi)
+    assert not np.any((codes != ref_codes) & ~near)
+
+
+def test_affine_all_general_boxes():
+    p = model("small_tanh")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_all")
+    rng = np.random.default_rng(8)
+    c = rng.uniform(-1, 1, size=(300, 3)).astype(np.float32)
+    v = (rng.normal(size=(300, 2, 3)) * 0.05).astype(np.float32)
+    _, bounds = func.classify_general_box(p, torch.as_tensor(c, device="cuda"),
+                                          torch.as_tensor(v, device="cuda"), return_bounds=True)
+    r_lo, r_hi = ofuncs.OracleFunc(p, "affine_all").bounds_general(c, v)
+    assert np.all(rel_close(bounds.cpu().numpy()[:, 0], r_lo, 1e-4))
+    assert np.all(rel_close(bounds.cpu().numpy()[:, 1], r_hi, 1e-4))
+
+
+def test_tree_affine_all_c3_matches_oracle():
+    p = model("c3_tanh_128x8")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_all")
+    out, st = kd_tree.build_tree(func, p, LO, UP, split_depth=8, with_interior_nodes=True,
+                                 with_exterior_nodes=True)
+    ref = otree.construct_uniform_unknown_levelset_tree(ofuncs.OracleFunc(p, "affine_all"), LO, UP,
+                                                        split_depth=8, with_interior_nodes=True,
+                                                        with_exterior_nodes=True)
+    note(f"[tree c3 affine_all depth 8] gpu {st['n_classified']} boxes {st['n_unknown']} unknown; "
+         f"oracle {ref['stats']['n_classified']} / {ref['stats']['n_unknown']}")
+    for key in ("unknown", "interior", "exterior"):
+        v = out[f"{key}_node_valid"].cpu().numpy()
+        np.testing.assert_array_equal(v, ref[f"{key}_node_valid"])
+        np.testing.assert_array_equal(out[f"{key}_node_lower"].cpu().numpy()[v], ref[f"{key}_node_lower"][v])
