@@ -20,6 +20,10 @@ def implicit_from_params(params, mode, **kwargs):
     elif mode == 'affine_truncate':
         ctx = affine.AffineContext('affine_truncate', truncate_count=kwargs['affine_n_truncate'],
                                    truncate_policy=kwargs['affine_truncate_policy'])
+        if ctx.truncate_policy != 'absolute':
+            # src/affine.py:146 divides a (v,) vector by a (w,) vector, which
+            # only broadcasts when v == w; only 'absolute' is defined
+            raise ValueError("only the 'absolute' truncate policy is supported")
     elif mode == 'affine_append':
         ctx = affine.AffineContext('affine_append', n_append=kwargs['affine_n_append'])
     elif mode == 'affine_all':

@@ -418,3 +418,27 @@ def test_tree_affine_all_c3_matches_oracle():
         v = out[f"{key}_node_valid"].cpu().numpy()
         np.testing.assert_array_equal(v, ref[f"{key}_node_valid"])
         np.testing.assert_array_equal(out[f"{key}_node_lower"].cpu().numpy()[v], ref[f"{key}_node_lower"][v])
+
+
+@pytest.mark.parametrize("mode,kw", [("affine_truncate", {"affine_n_truncate": 8, "affine_truncate_policy": "absolute"}),
+                                     ("affine_append", {"affine_n_append": 4})])
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "small_tanh", "fox"])
+def test_truncate_append_match_oracle(mode, kw, name):
+    note(f"[{mode}] {name}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, mode, **kw)
+    lo, up = boxes(600, seed=6, scale=0.2)
+    codes, bounds = func.classify_box(p, torch.as_tensor(lo, device="cuda"),
+                                      torch.as_tensor(up, device="cuda"), return_bounds=True)
+    codes, bounds = codes.cpu().numpy(), bounds.cpu().numpy()
+    r_lo, r_hi = ofuncs.OracleFunc(p, mode, np.float32, **kw).bounds_box(lo, up)
+    d_lo, d_hi = ofuncs.OracleFunc(p, mode, np.float64, **kw).bounds_box(lo, up)
+    noise_lo, noise_hi = np.abs(r_lo - d_lo), np.abs(r_hi - d_hi)
+    e_lo, e_hi = np.abs(bounds[:, 0] - r_lo), np.abs(bounds[:, 1] - r_hi)
+    scale = np.abs(d_lo) + np.abs(d_hi)
+    # row selection can flip on near-ties between rows; such cells are counted
+    bad = (e_lo > 1e-5 * np.max(np.abs(r_lo)) + 2 * noise_lo) | (e_hi > 1e-5 * np.max(np.abs(r_hi)) + 2 * noise_hi)
+    note(f"  max rel bound dev {np.max(np.maximum(e_lo, e_hi) / scale):.2e}, cells off the bar {bad.sum()}")
+    assert bad.mean() <= 0.01
+    ref_codes = oaff.sign_code(r_lo, r_hi, 0.0)
+    assert np.mean(codes != ref_codes) <= 0.01
