@@ -1,0 +1,166 @@
+// Device-resident level-set tree build.
+#include "niq_internal.h"
+
+extern "C" {
+
+// ----------------------------------------------------------- tree build
+namespace {
+
+struct TreeLayout {
+  long long cap;      // max nodes in any level list
+  long long nbmax;    // max batches per level
+  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], off_recheck, total;
+};
+
+niq_status tree_layout(const niq_tree_params* p, TreeLayout* L) {
+  if (!p) return fail(NIQ_ERR_INVALID, "null params");
+  const int B = p->batch_process_size;
+  if (B < 1 || (B & (B - 1)) || B > (1 << 24))
+    return fail(NIQ_ERR_INVALID, "batch_process_size must be a power of 2 (a factor of the bucket sizes)");
+  if (p->split_depth < 0 && p->node_terminate_thresh <= 0)
+    return fail(NIQ_ERR_INVALID, "must specify at least one of node_terminate_thresh or split_depth");
+  long long cap = LLONG_MAX;
+  if (p->split_depth >= 0) {
+    if (p->split_depth > 30) return fail(NIQ_ERR_UNSUPPORTED, "split_depth above 30");
+    cap = 1LL << p->split_depth;
+  }
+  if (p->node_terminate_thresh > 0) cap = std::min(cap, 2LL * p->node_terminate_thresh);
+  cap = std::max(cap, 1LL);
+  if (cap > (1LL << 30)) return fail(NIQ_ERR_UNSUPPORTED, "tree level capacity above 2^30 nodes");
+  L->cap = cap;
+  L->nbmax = (cap + B - 1) / B;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 255) / 256 * 256; return r; };
+  L->off_counters = take(sizeof(TreeCounters));
+  L->off_codes = take(sizeof(int) * cap);
+  L->off_bcnt = take(sizeof(int) * 4 * L->nbmax);
+  L->off_boff = take(sizeof(int) * 4 * L->nbmax);
+  L->off_lvl[0] = take(sizeof(float) * 6 * cap);
+  L->off_lvl[1] = take(sizeof(float) * 6 * cap);
+  L->off_recheck = take(sizeof(int) * cap);
+  L->total = o;
+  return NIQ_OK;
+}
+
+__global__ void k_tree_init(TreeCounters* C, float* lo, float* up, float l0, float l1, float l2, float u0,
+                            float u1, float u2) {
+  if (threadIdx.x == 0) {
+    C->n[0] = 1; C->n[1] = 0; C->n_int = 0; C->n_ext = 0; C->n_classified = 0; C->overflow = 0;
+    C->recheck_cnt = 0; C->n_recheck = 0;
+    lo[0] = l0; lo[1] = l1; lo[2] = l2;
+    up[0] = u0; up[1] = u1; up[2] = u2;
+  }
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) { C->level_n[i] = 0; C->level_int[i] = 0; C->level_ext[i] = 0; }
+}
+
+}  // namespace
+
+niq_status niq_tree_workspace_bytes(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p,
+                                    size_t* bytes) {
+  (void)m; (void)cfg;
+  TreeLayout L;
+  if (niq_status st = tree_layout(p, &L)) return st;
+  if (bytes) *bytes = L.total;
+  return NIQ_OK;
+}
+
+niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
+                          size_t ws_bytes, niq_tree_result* res, void* stream) {
+  if (niq_status st = check_model(m, 3)) return st;
+  if (!res) return fail(NIQ_ERR_INVALID, "null result");
+  TreeLayout L;
+  if (niq_status st = tree_layout(p, &L)) return st;
+  if (!ws || ws_bytes < L.total)
+    return fail(NIQ_ERR_CAPACITY, "tree workspace too small: need " + std::to_string(L.total) + " bytes");
+  if (!res->unknown_lower || !res->unknown_upper || res->unknown_capacity < 0)
+    return fail(NIQ_ERR_INVALID, "unknown-node output buffers are required");
+  if (p->with_interior_nodes && (!res->interior_lower || !res->interior_upper))
+    return fail(NIQ_ERR_INVALID, "interior output buffers are required");
+  if (p->with_exterior_nodes && (!res->exterior_lower || !res->exterior_upper))
+    return fail(NIQ_ERR_INVALID, "exterior output buffers are required");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* base = static_cast<unsigned char*>(ws);
+  TreeCounters* C = reinterpret_cast<TreeCounters*>(base + L.off_counters);
+  int* codes = reinterpret_cast<int*>(base + L.off_codes);
+  int* bcnt = reinterpret_cast<int*>(base + L.off_bcnt);
+  int* boff = reinterpret_cast<int*>(base + L.off_boff);
+  float* lvl[2] = {reinterpret_cast<float*>(base + L.off_lvl[0]), reinterpret_cast<float*>(base + L.off_lvl[1])};
+  const long long cap = L.cap;
+  const int B = p->batch_process_size;
+
+  hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, C, lvl[0], lvl[0] + 3 * cap, p->lower[0], p->lower[1],
+                     p->lower[2], p->upper[0], p->upper[1], p->upper[2]);
+  HIPCHK(hipGetLastError());
+
+  const int n_levels = p->split_depth >= 0 ? p->split_depth + 1 : 64;
+  int level = 0;
+  for (; level < std::min(n_levels, 62); ++level) {
+    bool quit_next = (level + 1 == n_levels);
+    if (p->node_terminate_thresh > 0) {
+      int n_host = 0;
+      HIPCHK(hipMemcpyAsync(&n_host, &C->n[level & 1], sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      quit_next = quit_next || (n_host >= p->node_terminate_thresh);
+    }
+    const int split = quit_next ? 0 : 1;
+    const long long max_n = level < 40 ? std::min(cap, 1LL << level) : cap;
+    float* cur_lo = lvl[level & 1];
+    float* cur_up = cur_lo + 3 * cap;
+
+    ClassifyArgs a{};
+    a.lower = cur_lo; a.upper = cur_up; a.d = 3; a.n_dev = &C->n[level & 1]; a.offset = p->offset;
+    a.codes = codes;
+    if (cfg && cfg->recheck_tol > 0.f) {
+      a.recheck_tol = cfg->recheck_tol;
+      a.recheck_cnt = &C->recheck_cnt;
+      a.recheck_idx = reinterpret_cast<int*>(base + L.off_recheck);
+      a.recheck_cap = (int)std::min<long long>(cap, INT_MAX);
+    }
+    if (niq_status st = classify_dispatch(m, cfg, 3, a, max_n, s)) return st;
+
+    const unsigned nb = (unsigned)((max_n + B - 1) / B);
+    hipLaunchKernelGGL(k_tree_count, dim3(nb), dim3(256), 0, s, codes, C, level, B, bcnt);
+    HIPCHK(hipGetLastError());
+
+    TreeScatterArgs sa{};
+    sa.codes = codes; sa.lower = cur_lo; sa.upper = cur_up;
+    if (split) {
+      sa.next_lower = lvl[(level + 1) & 1];
+      sa.next_upper = sa.next_lower + 3 * cap;
+      sa.cap_next = cap;
+    } else {
+      sa.next_lower = res->unknown_lower;
+      sa.next_upper = res->unknown_upper;
+      sa.cap_next = res->unknown_capacity;
+    }
+    if (p->with_interior_nodes) { sa.int_lower = res->interior_lower; sa.int_upper = res->interior_upper; sa.cap_int = res->interior_capacity; }
+    if (p->with_exterior_nodes) { sa.ext_lower = res->exterior_lower; sa.ext_upper = res->exterior_upper; sa.cap_ext = res->exterior_capacity; }
+    const long long cap_int = p->with_interior_nodes ? res->interior_capacity : LLONG_MAX;
+    const long long cap_ext = p->with_exterior_nodes ? res->exterior_capacity : LLONG_MAX;
+    hipLaunchKernelGGL(k_tree_scan, dim3(1), dim3(1024), 0, s, bcnt, boff, C, level, B, split, sa.cap_next, cap_int,
+                       cap_ext);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_tree_scatter, dim3(nb), dim3(256), 0, s, sa, bcnt, boff, C, level, B, split);
+    HIPCHK(hipGetLastError());
+    if (quit_next) break;
+  }
+  TreeCounters hc;
+  HIPCHK(hipMemcpyAsync(&hc, C, sizeof(TreeCounters), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  res->n_levels = level + 1;
+  res->n_unknown = hc.n[(level + 1) & 1];
+  res->n_interior = hc.n_int;
+  res->n_exterior = hc.n_ext;
+  res->n_classified = hc.n_classified;
+  res->n_recheck = hc.n_recheck;
+  for (int i = 0; i < 64; ++i) {
+    res->level_n[i] = hc.level_n[i];
+    res->level_interior[i] = hc.level_int[i];
+    res->level_exterior[i] = hc.level_ext[i];
+  }
+  if (hc.overflow) return fail(NIQ_ERR_CAPACITY, "a tree output buffer was too small");
+  return NIQ_OK;
+}
+
+
+}  // extern "C"

@@ -130,25 +130,32 @@ __device__ __forceinline__ void affine_rule_rigorous(int act, float* v) {
   const float lo = v[0] - rad, hi = v[0] + rad;
   float a, b, d;
   act_rule(act, lo, hi, a, b, d);
-  // the error of the chosen line on [lo, hi], re-evaluated and padded for
-  // the rounding of alpha, beta and the transcendental evaluations
-  float m;
+  if (b == 0.f && d == 0.f && (a == 1.f || a == 0.f)) {
+    // determinate feature (identity or zero): exact in FP32, no margin needed
+#pragma unroll
+    for (int i = 0; i <= V + 1; ++i) v[i] = a * v[i];
+    return;
+  }
+  // The chosen line's error on [lo, hi], re-evaluated, plus margins for the
+  // rounding of alpha x + beta and of the activation values (|act(x)| <= |x|
+  // for relu, elu, tanh and sin, so both scale with |lo| + |hi|).
+  const float span = fabsf(lo) + fabsf(hi);
   if (act == ACT_RELU) {
     const float e0 = fabsf(relu_t(lo) - (a * lo + b));
     const float e1 = fabsf(relu_t(hi) - (a * hi + b));
     d = fmaxf(fmaxf(fabsf(d), fabsf(b)), fmaxf(e0, e1));
-    m = 16.f * kU24 * (fabsf(lo) + fabsf(hi));
   } else {
     d = fabsf(d);
-    m = 64.f * kU24 * (fabsf(b) + a * (fabsf(lo) + fabsf(hi)) + 1.f);
   }
-  const float base = a * v[0] + b;
+  const float m = 8.f * kU24 * (span + fabsf(b));
+  const float pa = a * v[0];
+  const float base = pa + b;
   float amag = 0.f;
   v[0] = base;
 #pragma unroll
   for (int i = 0; i < V; ++i) { v[1 + i] = a * v[1 + i]; amag = amag + fabsf(v[1 + i]); }
-  // err' = a err + delta, plus the rounding of base' and aff'
-  const float err = a * v[V + 1] + d + m + 4.f * kU24 * (fabsf(base) + amag);
+  // err' = a err + delta + margins for the rounding of base' and aff'
+  const float err = a * v[V + 1] + d + m + 2.f * kU24 * (fabsf(pa) + fabsf(b) + amag);
   v[V + 1] = up1(up1(err * (1.f + 4.f * kU24)));
 }
 

@@ -43,7 +43,7 @@ struct CpArgs {
 
 // Pop the top B items into the batch buffers and form their sample points
 // (src/kd_tree.py:679-686, 703-704).
-__global__ void __launch_bounds__(256) k_cp_gather(CpState* S, CpArgs a) {
+static __global__ void __launch_bounds__(256) k_cp_gather(CpState* S, CpArgs a) {
   const int top = S->top;
   const int pop = max(top - a.B, 0);
   const int n = top - pop;
@@ -77,7 +77,7 @@ __device__ __forceinline__ float norm3(float x, float y, float z) { return sqrtf
 
 // One workgroup: distances, scatter-min, location, compaction and push
 // (src/kd_tree.py:692-760).
-__global__ void __launch_bounds__(1024) k_cp_update(CpState* S, CpArgs a) {
+static __global__ void __launch_bounds__(1024) k_cp_update(CpState* S, CpArgs a) {
   const int n = S->n;
   const int pop = S->pop;
   if (n == 0) return;
@@ -182,7 +182,7 @@ struct McTable { signed char tri[256][16]; };
 
 // Grid points of leaves [l0, l0 + nl): (2^s + 1)^3 per leaf, 'ij' order,
 // coordinates lower + i * (upper - lower) / 2^s, the last one = upper.
-__global__ void __launch_bounds__(256) k_mc_grid(const float* __restrict__ lo, const float* __restrict__ up,
+static __global__ void __launch_bounds__(256) k_mc_grid(const float* __restrict__ lo, const float* __restrict__ up,
                                                long long l0, long long nl, int s, float* __restrict__ pts) {
   const int np = (1 << s) + 1;
   const long long total = nl * np * np * np;
@@ -219,7 +219,7 @@ __device__ __forceinline__ int mc_case(const float* vals, long long leaf_rel, in
 }
 
 // Per-leaf triangle counts (one workgroup per leaf, threads over subcells).
-__global__ void __launch_bounds__(256) k_mc_count(const McTable* __restrict__ T, const float* __restrict__ vals,
+static __global__ void __launch_bounds__(256) k_mc_count(const McTable* __restrict__ T, const float* __restrict__ vals,
                                                 long long nl, int s, long long* __restrict__ cnt) {
   const int nsub = 1 << (3 * s);
   __shared__ int red[256];
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(256) k_mc_count(const McTable* __restrict__ T,
 }
 
 // Exclusive prefix sum of n int64 counts in one workgroup (chunked).
-__global__ void __launch_bounds__(1024) k_scan_i64(const long long* __restrict__ in, long long n,
+static __global__ void __launch_bounds__(1024) k_scan_i64(const long long* __restrict__ in, long long n,
                                                  long long base, long long* __restrict__ out,
                                                  long long* __restrict__ total) {
   __shared__ long long s[1024];
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(1024) k_scan_i64(const long long* __restrict__
 }
 
 // Emit the triangles of each leaf at its offset, in (subcell, slot) order.
-__global__ void __launch_bounds__(256) k_mc_emit(const McTable* __restrict__ T, const float* __restrict__ vals,
+static __global__ void __launch_bounds__(256) k_mc_emit(const McTable* __restrict__ T, const float* __restrict__ vals,
                                                const float* __restrict__ lo, const float* __restrict__ up,
                                                long long l0, long long nl, int s,
                                                const long long* __restrict__ off, float* __restrict__ tri,

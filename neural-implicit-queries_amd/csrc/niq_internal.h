@@ -1,0 +1,139 @@
+#pragma once
+// Shared host-side pieces of the C ABI (include/niq.h): error state, launch
+// geometry helpers and the model handle.
+#include "niq.h"
+#include "niq_kernels.h"
+#include "niq_bf16.h"
+
+
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace niqi {
+
+extern thread_local std::string g_err;
+
+inline niq_status fail(niq_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      return fail(NIQ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+inline int round16(int x) { return (x + 15) / 16 * 16; }
+inline int round32(int x) { return (x + 31) / 32 * 32; }
+
+// host bf16 conversions (same bit rules as the device)
+inline unsigned short h_bf16_rn(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+inline float h_bf16_f(unsigned short h) {
+  unsigned u = (unsigned)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+inline unsigned short h_bf16_up_nonneg(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  unsigned short h = (unsigned short)(u >> 16);
+  if ((u & 0xffffu) && (u & 0x7fffffffu) < 0x7f800000u) h = (unsigned short)(h + 1);
+  return h;
+}
+// gamma_n(u) = n u / (1 - n u), rounded up generously
+inline double gamma_n(double n, double u) { return n * u / (1.0 - n * u) * (1.0 + 1e-6); }
+
+// Template width that holds every padded layer width of a model.
+inline int template_width(int wmax) {
+  if (wmax <= 64) return 64;
+  if (wmax <= 128) return 128;
+  if (wmax <= 256) return 256;
+  if (wmax <= 512) return 512;
+  return -1;
+}
+
+inline int num_cus() {
+  static int cus = -1;
+  if (cus < 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+  }
+  return cus;
+}
+
+// Persistent grid: resident workgroups of kernel `f` over the whole device.
+template <class F>
+inline long long persistent_blocks(F f, int threads) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find((const void*)f);
+  int occ = 0;
+  if (it == cache.end()) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, threads, 0) != hipSuccess || occ < 1) occ = 1;
+    cache[(const void*)f] = occ;
+  } else {
+    occ = it->second;
+  }
+  return (long long)occ * num_cus();
+}
+
+inline unsigned grid_for(long long work_blocks, long long persist) {
+  long long g = std::min(work_blocks, persist);
+  return (unsigned)std::max(1LL, g);
+}
+
+}  // namespace niqi
+
+using namespace niqi;
+
+struct niq_model {
+  int n_layers = 0;
+  int in_dim = 0;
+  int wt = 0;                  // template width
+  int wt16 = 0;                // template width of the precision-mode copy
+  DevModelBf16 host16;         // precision-mode copy (bf16 hi/lo/|W| weights)
+  DevModelBf16* dev16 = nullptr;
+  void* weights16 = nullptr;
+  std::vector<int> dims, acts;
+  DevModel host;               // with device pointers
+  DevModel* dev = nullptr;     // device copy of `host`
+  void* weights = nullptr;     // packed weights and biases
+  size_t bytes = 0;
+};
+
+namespace niqi {
+
+inline niq_status check_model(const niq_model* m, int d) {
+  if (!m) return fail(NIQ_ERR_INVALID, "null model");
+  if (d != m->in_dim) return fail(NIQ_ERR_INVALID, "point dimension does not match the model input");
+  return NIQ_OK;
+}
+
+// Classification of up to max_n boxes (count may be device-resident), routed
+// by (mode, precision, box vectors).  Defined in api_classify.hip.
+niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec, const ClassifyArgs& a,
+                             long long max_n, hipStream_t s);
+// ... with the optional FP64 re-test list in stream-ordered scratch.
+niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, int nvec, ClassifyArgs a,
+                                 long long n, hipStream_t s);
+
+}  // namespace niqi
+

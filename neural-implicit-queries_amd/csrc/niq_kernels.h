@@ -106,7 +106,7 @@ k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
 // FP32 bound lies within recheck_tol * scale of the threshold, so that a cell
 // is only reported empty when the FP64 restatement agrees.
 template <int KIND, int V, int WMAX>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_recheck_fp64(const DevModel* __restrict__ Mp, ClassifyArgs a) {
   using PL = Planes<KIND, V, 0>;
   constexpr int R = PL::R;
@@ -231,7 +231,7 @@ struct TreeCounters {
 };
 
 // Per batch of B nodes: counts of UNKNOWN / POSITIVE / NEGATIVE codes.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_tree_count(const int* __restrict__ codes, const TreeCounters* __restrict__ C, int level, int B,
              int* __restrict__ bcnt) {
   const int n = C->n[level & 1];
@@ -263,7 +263,7 @@ k_tree_count(const int* __restrict__ codes, const TreeCounters* __restrict__ C, 
 
 // Single workgroup: exclusive scan of the per-batch counts, level totals.
 // boff[b] = {unknown prefix, exterior offset, interior offset}.
-__global__ void __launch_bounds__(1024)
+static __global__ void __launch_bounds__(1024)
 k_tree_scan(const int* __restrict__ bcnt, int* __restrict__ boff, TreeCounters* C, int level, int B,
             int split, long long cap_next, long long cap_int, long long cap_ext) {
   const int n = C->n[level & 1];
@@ -329,7 +329,7 @@ __device__ __forceinline__ void store3(float* p, long long i, float x, float y, 
 
 // Stable per-batch placement of children and finished leaves, in the
 // reference's order (src/kd_tree.py:46-96, src/bucketing.py:16-32).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __restrict__ boff,
                const TreeCounters* __restrict__ C, int level, int B, int split) {
   const int n = C->n[level & 1];

@@ -282,6 +282,26 @@ def test_bf16_mode_is_sound_and_close(name, mode):
     note(f"  false-empty {false_empty.sum()}, width inflation vs FP64: median {np.median(infl):.6f} "
          f"max {np.max(infl):.6f}, determinate {np.mean(codes != 0):.3f} vs FP64 {np.mean(c64 != 0):.3f}")
     assert not np.any(false_empty)
+    # tiny boxes (half-widths down to 2e-4) are included: widening is relative
+    # to |X| |W|, so it shows most where the true radius is smallest
+    assert np.median(infl) < 1.15
+
+
+@pytest.mark.parametrize("name", ["c4_relu_512x8", "fox", "bunny"])
+def test_bf16_inflation_at_finest_tree_cells(name):
+    """At the finest cells of a depth-21 tree (half-width 1/128) the median
+    width inflation of the precision mode over FP64 stays within 5 %."""
+    p = model(name)
+    f16, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision="bf16")
+    rng = np.random.default_rng(4)
+    c = rng.uniform(-1, 1, size=(300, 3))
+    lo, up = (c - 1 / 128).astype(np.float32), (c + 1 / 128).astype(np.float32)
+    _, bounds = f16.classify_box(p, torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda"),
+                                 return_bounds=True)
+    bounds = bounds.cpu().numpy()
+    d_lo, d_hi = ofuncs.OracleFunc(p, "affine_fixed", np.float64).bounds_box(lo, up)
+    infl = (bounds[:, 1] - bounds[:, 0]) / (d_hi - d_lo)
+    note(f"[bf16 inflation at h=1/128] {name}: median {np.median(infl):.5f} p90 {np.percentile(infl, 90):.5f}")
     assert np.median(infl) < 1.05
 
 
