@@ -60,7 +60,10 @@ typedef struct niq_eval_cfg {
   int truncate_count;  /* affine_truncate: rows kept */
   int n_append;        /* affine_append: rows appended per activation */
   float recheck_tol;   /* > 0: FP64 re-test of cells within tol*scale of the threshold */
-  int reserved[3];
+  int variant;         /* kernel variant, 0 = default; precision mode: 1 streams
+                          the packed bf16 weight copies, 2 splits FP32 weights
+                          in registers (the default) */
+  int reserved[2];
 } niq_eval_cfg;
 
 /* ---- model ------------------------------------------------------------
@@ -136,6 +139,22 @@ niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg,
                           const niq_tree_params* p, void* workspace, size_t workspace_bytes,
                           niq_tree_result* res, void* stream);
 
+/* Asynchronous form of niq_tree_build for serving loops (the same kernels and
+ * results as src/extract_cell.py:95-160 with split_depth set): enqueues every
+ * level on `stream` and returns without synchronising or allocating, so the
+ * call can be captured in a hipGraph.  Requires split_depth >= 0, no
+ * node_terminate_thresh and a fused-kernel mode (interval, affine_fixed,
+ * slope_interval; FP32 with optional FP64 re-test, or BF16).  The tree's
+ * counters stay in the workspace at *counters_offset; after the stream has
+ * finished, copy niq_tree_counters_bytes() from there to the host and call
+ * niq_tree_decode to fill the counts of `res`. */
+niq_status niq_tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg,
+                            const niq_tree_params* p, void* workspace, size_t workspace_bytes,
+                            niq_tree_result* res, size_t* counters_offset, void* stream);
+size_t niq_tree_counters_bytes(void);
+niq_status niq_tree_decode(const void* host_counters, const niq_tree_params* p,
+                           niq_tree_result* res);
+
 /* ---- ray casting ------------------------------------------------------------
  * Replaces cast_rays / cast_rays_iter (src/queries.py:134-175, 39-132) with the
  * options of get_default_cast_opts (src/queries.py:23-36). */
@@ -182,6 +201,63 @@ niq_status niq_mc_extract(const niq_model* m, const float* leaf_lower, const flo
 /* The marching-cubes triangle table the kernels use: out[256 * 16] edge ids,
  * -1 padded (host only; for checking). */
 int niq_mc_table(signed char* out);
+
+/* ---- serving loop ------------------------------------------------------------
+ * No counterpart in the reference (it runs one query at a time from one Python
+ * thread, src/queries.py:134, src/kd_tree.py:103,766).  A server owns
+ * n_streams worker threads, each with its own HIP stream at the default
+ * priority.  Requests queue in arrival order; a worker takes the oldest one as
+ * soon as its device memory fits under mem_cap_bytes beside the work in
+ * flight.  Ray requests waiting in the queue with the same models, eval
+ * config and options are merged into the cast of the request taken (up to
+ * max_merge_rays rays): every ray is independent, so merged results equal
+ * lone ones bit for bit.  Tree builds in the fused-kernel modes with a
+ * split_depth replay a hipGraph captured once per (model, config, depth,
+ * capacities) on each worker.  Closest-point batches run as they are.
+ *
+ * Buffers: inputs must be complete on the device when submitted and stay
+ * alive, and outputs untouched, until the request has been polled done.
+ * Model handles must outlive the requests that use them. */
+typedef struct niq_server niq_server;
+
+typedef struct niq_serve_info {
+  int state;             /* 0 queued, 1 running, 2 done */
+  niq_status status;     /* the request's own status once done */
+  double queue_ms;       /* submit -> start */
+  double latency_ms;     /* submit -> done */
+  int64_t n_evals;       /* rays: N_evals of the cast that served the request */
+  int32_t n_batched;     /* rays: requests merged into that cast */
+  int32_t graph_replay;  /* trees: 1 if served by replaying a captured graph */
+  int64_t n_processed;   /* closest point: nodes processed */
+} niq_serve_info;
+
+niq_status niq_serve_create(int n_streams, size_t mem_cap_bytes, int64_t max_merge_rays,
+                            niq_server** out);
+/* Waits for every accepted request, then stops the workers. */
+niq_status niq_serve_destroy(niq_server* srv);
+
+niq_status niq_serve_submit_rays(niq_server* srv, const niq_model* const* models, int n_models,
+                                 const niq_eval_cfg* cfg, const float* roots, const float* dirs,
+                                 int64_t n, const niq_cast_opts* opts, float* out_t,
+                                 int32_t* out_hit_id, int32_t* out_count, uint64_t* request_id);
+/* The result's buffers and capacities are those of niq_tree_build; counts are
+ * returned by niq_serve_poll. */
+niq_status niq_serve_submit_tree(niq_server* srv, const niq_model* m, const niq_eval_cfg* cfg,
+                                 const niq_tree_params* p, const niq_tree_result* res,
+                                 uint64_t* request_id);
+niq_status niq_serve_submit_closest_point(niq_server* srv, const niq_model* m, const niq_eval_cfg* cfg,
+                                          const float lower[3], const float upper[3],
+                                          const float* query_points, int64_t Q, float eps,
+                                          int batch_process_size, float* out_dist, float* out_loc,
+                                          uint64_t* request_id);
+/* Fills *info (and, for a tree, the counts of *tree_res; may be NULL).  With
+ * wait != 0 blocks until the request is done.  A request polled done is
+ * forgotten: a later poll of its id returns NIQ_ERR_INVALID.  When the
+ * request failed, niq_last_error() on the polling thread gives its message. */
+niq_status niq_serve_poll(niq_server* srv, uint64_t request_id, int wait, niq_serve_info* info,
+                          niq_tree_result* tree_res);
+/* Device bytes the server holds now: admitted requests plus cached graphs. */
+size_t niq_serve_bytes_in_use(niq_server* srv);
 
 /* ---- diagnostics -------------------------------------------------------- */
 const char* niq_last_error(void);

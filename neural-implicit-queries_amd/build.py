@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "niq", "libniq.so")
-SOURCES = ["api_model.hip", "api_classify.hip", "api_tree.hip", "api_rays.hip", "api_cp_mc.hip"]
+SOURCES = ["api_model.hip", "api_classify.hip", "api_tree.hip", "api_rays.hip", "api_cp_mc.hip",
+           "api_serve.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -45,7 +46,7 @@ def build(verbose=True):
             raise RuntimeError(f"hipcc failed on {src} ({p.returncode}):\n{_clean(err)}")
         objs.append(obj)
     tmp = OUT + ".tmp"
-    res = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs,
+    res = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs,
                          capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed ({res.returncode}):\n{_clean(res.stderr)}")

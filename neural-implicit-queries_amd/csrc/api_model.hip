@@ -105,7 +105,7 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
       const size_t ip = round32(dims[i]);
       const size_t op = (i + 1 < n_layers) ? round32(dims[i + 1]) : round16(dims[i + 1]);
       o16[i] = t16;
-      t16 += 4 * ip * op * sizeof(unsigned short);
+      t16 += 4 * ip * op * sizeof(unsigned short) + ip * op * sizeof(float);
       t16 = (t16 + 255) / 256 * 256;
       ob16[i] = t16;
       t16 += op * sizeof(float);
@@ -126,6 +126,7 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
       unsigned short* w2p = w1p + nfrag;
       unsigned short* w3p = w2p + nfrag;
       unsigned short* wap = w3p + nfrag;
+      float* w32p = reinterpret_cast<float*>(wap + nfrag);
       for (int ft = 0; ft < op / 16; ++ft)
         for (int kb = 0; kb < nkb; ++kb)
           for (int l = 0; l < 64; ++l)
@@ -141,6 +142,12 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
               w2p[idx] = b2;
               w3p[idx] = h_bf16_rn(r1 - h_bf16_f(b2));
               wap[idx] = h_bf16_up_nonneg(std::fabs(w));
+              // FP32 copy: [fragment][half][lane][4]
+              const size_t i32 = (((size_t)(ft * nkb + kb) * 2 + sl / 4) * 64 + l) * 4 + sl % 4;
+              w32p[i32] = w;
+              // bf16 terms of FP32 subnormals are not guaranteed by the
+              // hardware conversions: the precision mode refuses such weights
+              if (w != 0.f && std::fabs(w) < 1e-30f) m->bf16_ok = false;
             }
       float* bp = reinterpret_cast<float*>(h16.data() + ob16[i]);
       for (int f = 0; f < outd; ++f) bp[f] = (b && b[i]) ? b[i][f] : 0.f;
@@ -158,6 +165,9 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
       L.c_base = (float)((4.1 * std::ldexp(1.0, -24) + g_chunk + g_sum) * 1.5);
       L.c_aff = (float)((3.1 * std::ldexp(1.0, -16) + g_chunk + g_sum) * 1.5);
       L.e_fix = (float)(1.0 + 1.5 * (gamma_n(33.0, std::ldexp(1.0, -23)) + g_sum));
+      // e |w1| + e |w2| in one 64-term chunk, and |W| <= (|w1| + |w2|) / (1 - 2^-16)
+      L.e_fix32 = (float)((1.0 + 1.5 * (gamma_n(65.0, std::ldexp(1.0, -23)) + g_sum)) *
+                          (1.0 + std::ldexp(1.0, -15)));
     }
     m->host16.wmax = wmax16;
     m->wt16 = template_width(wmax16);
@@ -176,6 +186,7 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
       L.W2 = L.W1 + nfrag;
       L.W3 = L.W2 + nfrag;
       L.Wa = L.W3 + nfrag;
+      L.W32 = reinterpret_cast<const floatx4*>(L.Wa + nfrag);
       L.bias = reinterpret_cast<const float*>(static_cast<unsigned char*>(m->weights16) + ob16[i]);
     }
     e = hipMemcpy(m->dev16, &m->host16, sizeof(DevModelBf16), hipMemcpyHostToDevice);

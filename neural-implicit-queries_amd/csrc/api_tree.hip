@@ -42,11 +42,18 @@ niq_status tree_layout(const niq_tree_params* p, TreeLayout* L) {
   return NIQ_OK;
 }
 
+// Root box from the arguments, or from `root` (6 floats on the device:
+// lower then upper) when it is given, so a captured build can be replayed for
+// any root box.
 __global__ void k_tree_init(TreeCounters* C, float* lo, float* up, float l0, float l1, float l2, float u0,
-                            float u1, float u2) {
+                            float u1, float u2, const float* root) {
   if (threadIdx.x == 0) {
     C->n[0] = 1; C->n[1] = 0; C->n_int = 0; C->n_ext = 0; C->n_classified = 0; C->overflow = 0;
     C->recheck_cnt = 0; C->n_recheck = 0;
+    if (root) {
+      l0 = root[0]; l1 = root[1]; l2 = root[2];
+      u0 = root[3]; u1 = root[4]; u2 = root[5];
+    }
     lo[0] = l0; lo[1] = l1; lo[2] = l2;
     up[0] = u0; up[1] = u1; up[2] = u2;
   }
@@ -64,8 +71,31 @@ niq_status niq_tree_workspace_bytes(const niq_model* m, const niq_eval_cfg* cfg,
   return NIQ_OK;
 }
 
-niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
-                          size_t ws_bytes, niq_tree_result* res, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Decode the device counters of a finished build into `res`.
+void decode_counters(const TreeCounters& hc, int n_levels, niq_tree_result* res) {
+  res->n_levels = n_levels;
+  res->n_unknown = hc.n[n_levels & 1];
+  res->n_interior = hc.n_int;
+  res->n_exterior = hc.n_ext;
+  res->n_classified = hc.n_classified;
+  res->n_recheck = hc.n_recheck;
+  for (int i = 0; i < 64; ++i) {
+    res->level_n[i] = hc.level_n[i];
+    res->level_interior[i] = hc.level_int[i];
+    res->level_exterior[i] = hc.level_ext[i];
+  }
+}
+
+// Enqueue every level of a build on `stream`.  With sync_allowed == false the
+// call neither synchronises nor allocates (so it can be captured in a
+// hipGraph); it then requires split_depth and a fused-kernel mode.
+niq_status tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
+                        size_t ws_bytes, niq_tree_result* res, void* stream, bool sync_allowed, int* levels_out,
+                        size_t* counters_offset, const float* root_dev) {
   if (niq_status st = check_model(m, 3)) return st;
   if (!res) return fail(NIQ_ERR_INVALID, "null result");
   TreeLayout L;
@@ -89,14 +119,14 @@ niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq
   const int B = p->batch_process_size;
 
   hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, C, lvl[0], lvl[0] + 3 * cap, p->lower[0], p->lower[1],
-                     p->lower[2], p->upper[0], p->upper[1], p->upper[2]);
+                     p->lower[2], p->upper[0], p->upper[1], p->upper[2], root_dev);
   HIPCHK(hipGetLastError());
 
   const int n_levels = p->split_depth >= 0 ? p->split_depth + 1 : 64;
   int level = 0;
   for (; level < std::min(n_levels, 62); ++level) {
     bool quit_next = (level + 1 == n_levels);
-    if (p->node_terminate_thresh > 0) {
+    if (p->node_terminate_thresh > 0 && sync_allowed) {
       int n_host = 0;
       HIPCHK(hipMemcpyAsync(&n_host, &C->n[level & 1], sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
@@ -144,23 +174,58 @@ niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq
     HIPCHK(hipGetLastError());
     if (quit_next) break;
   }
+  if (levels_out) *levels_out = level + 1;
+  if (counters_offset) *counters_offset = L.off_counters;
+  return NIQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+niq_status niq_tree_build(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
+                          size_t ws_bytes, niq_tree_result* res, void* stream) {
+  int levels = 0;
+  size_t off = 0;
+  if (niq_status st = tree_enqueue(m, cfg, p, ws, ws_bytes, res, stream, true, &levels, &off, nullptr)) return st;
+  hipStream_t s = (hipStream_t)stream;
   TreeCounters hc;
-  HIPCHK(hipMemcpyAsync(&hc, C, sizeof(TreeCounters), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&hc, static_cast<unsigned char*>(ws) + off, sizeof(TreeCounters), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  res->n_levels = level + 1;
-  res->n_unknown = hc.n[(level + 1) & 1];
-  res->n_interior = hc.n_int;
-  res->n_exterior = hc.n_ext;
-  res->n_classified = hc.n_classified;
-  res->n_recheck = hc.n_recheck;
-  for (int i = 0; i < 64; ++i) {
-    res->level_n[i] = hc.level_n[i];
-    res->level_interior[i] = hc.level_int[i];
-    res->level_exterior[i] = hc.level_ext[i];
-  }
+  decode_counters(hc, levels, res);
   if (hc.overflow) return fail(NIQ_ERR_CAPACITY, "a tree output buffer was too small");
   return NIQ_OK;
 }
 
+niq_status niq_tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
+                            size_t ws_bytes, niq_tree_result* res, size_t* counters_offset, void* stream) {
+  return niqi::tree_enqueue_async(m, cfg, p, ws, ws_bytes, res, counters_offset, nullptr, stream);
+}
+
+size_t niq_tree_counters_bytes(void) { return sizeof(TreeCounters); }
+
+niq_status niq_tree_decode(const void* host_counters, const niq_tree_params* p, niq_tree_result* res) {
+  if (!host_counters || !p || !res || p->split_depth < 0) return fail(NIQ_ERR_INVALID, "bad arguments");
+  TreeCounters hc;
+  std::memcpy(&hc, host_counters, sizeof(hc));
+  decode_counters(hc, p->split_depth + 1, res);
+  if (hc.overflow) return fail(NIQ_ERR_CAPACITY, "a tree output buffer was too small");
+  return NIQ_OK;
+}
 
 }  // extern "C"
+
+namespace niqi {
+
+niq_status tree_enqueue_async(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
+                              size_t ws_bytes, niq_tree_result* res, size_t* counters_offset, const float* root_dev,
+                              void* stream) {
+  if (!p || p->split_depth < 0 || p->node_terminate_thresh > 0)
+    return fail(NIQ_ERR_INVALID, "niq_tree_enqueue needs split_depth and no node_terminate_thresh");
+  const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
+  if (mode == NIQ_MODE_AFFINE_ALL || mode == NIQ_MODE_AFFINE_TRUNCATE || mode == NIQ_MODE_AFFINE_APPEND)
+    return fail(NIQ_ERR_UNSUPPORTED, "niq_tree_enqueue runs the fused-kernel modes only");
+  return tree_enqueue(m, cfg, p, ws, ws_bytes, res, stream, false, nullptr, counters_offset, root_dev);
+}
+
+}  // namespace niqi

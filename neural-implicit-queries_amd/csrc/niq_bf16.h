@@ -27,6 +27,8 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 // Packed bf16 weight fragments of one layer (K padded to 32): for fragment
 // (ft, kb) lane l holds 8 values W[32 kb + 8 (l / 16) + s][16 ft + l % 16].
+// The same fragments also exist once in FP32 (W32: [fragment][half][lane]
+// float4, half h holding s = 4h..4h+3), split into bf16 terms in registers.
 struct DevLayerBf16 {
   int in_pad;    // multiple of 32
   int out_pad;   // multiple of 16
@@ -36,12 +38,16 @@ struct DevLayerBf16 {
   const bf16x8* W2;   // w2 = bf16(W - w1)
   const bf16x8* W3;   // w3 = bf16(W - w1 - w2)
   const bf16x8* Wa;   // |W| rounded up to bf16
+  const floatx4* W32; // W itself (FP32), for the in-register split
   const float* bias;
   float c_base;       // a-priori contraction bound of the base row
   float c_aff;        // ... of the error-term rows
   float e_fix;        // 1 + bound of the (non-negative) e-plane accumulation
-  float pad2_;
+  float e_fix32;      // the same for e |w1| + e |w2| (in-register split)
 };
+
+// Weight sources of the precision-mode kernel.
+enum { WSRC_BF16 = 0, WSRC_FP32 = 1 };
 
 struct DevModelBf16 {
   int n_layers;
@@ -68,6 +74,43 @@ __device__ __forceinline__ __bf16 bits_to_bf16(unsigned short h) { return __buil
 // next float up (x finite and >= 0)
 __device__ __forceinline__ float up1(float x) {
   return (x >= 0.f && x < INFINITY) ? __uint_as_float(__float_as_uint(x) + 1u) : x;
+}
+
+typedef float floatx2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned uintx4_t __attribute__((ext_vector_type(4)));
+
+// Two floats to packed bf16, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ unsigned pk_bf16_rn(float a, float b) {
+  const floatx2_t x = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(x, bf16x2_t));
+}
+
+// In-register split of 8 FP32 weights into W = w1 + w2 + w3 exactly (each
+// term the bf16 rounding of the remainder, as the packed copies hold), plus
+// |w1| and |w2|: |W| <= (|w1| + |w2|) / (1 - 2^-16).
+__device__ __forceinline__ void split_w8(const floatx4 lo, const floatx4 hi, bf16x8& w1, bf16x8& w2, bf16x8& w3,
+                                         bf16x8& a1, bf16x8& a2) {
+  uintx4_t u1, u2, u3;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float x0 = q < 2 ? lo[2 * q] : hi[2 * q - 4];
+    const float x1 = q < 2 ? lo[2 * q + 1] : hi[2 * q - 3];
+    const unsigned h1 = pk_bf16_rn(x0, x1);
+    const float r0 = x0 - __uint_as_float(h1 << 16);
+    const float r1 = x1 - __uint_as_float(h1 & 0xffff0000u);
+    const unsigned h2 = pk_bf16_rn(r0, r1);
+    const float s0 = r0 - __uint_as_float(h2 << 16);
+    const float s1 = r1 - __uint_as_float(h2 & 0xffff0000u);
+    u1[q] = h1;
+    u2[q] = h2;
+    u3[q] = pk_bf16_rn(s0, s1);
+  }
+  w1 = __builtin_bit_cast(bf16x8, u1);
+  w2 = __builtin_bit_cast(bf16x8, u2);
+  w3 = __builtin_bit_cast(bf16x8, u3);
+  a1 = __builtin_bit_cast(bf16x8, u1 & 0x7fff7fffu);
+  a2 = __builtin_bit_cast(bf16x8, u2 & 0x7fff7fffu);
 }
 
 // LDS layout: X[plane][box][k] as bf16 with the 16-byte chunk index
@@ -162,7 +205,7 @@ __device__ __forceinline__ void affine_rule_rigorous(int act, float* v) {
 // Fused forward in precision mode (affine forms only: KIND 0).  Xs holds
 // PlanesBf16<V>::NPL bf16 planes.  fin(v, b) gets base, aff_i, err (FP32,
 // err rigorous) of output feature 0.
-template <int V, int NB, int NW, int WMAX, class Fin>
+template <int V, int NB, int NW, int WMAX, int WS, class Fin>
 __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned short* __restrict__ Xs, Fin&& fin) {
   using PB = PlanesBf16<V>;
   constexpr int NS = V + 1;
@@ -203,10 +246,15 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
           const int ft = wave + f * NW;
           if (ft >= nft) continue;
           const size_t o = (size_t)(ft * nkb + kb) * 64 + lane;
-          const bf16x8 w1 = Ly.W1[o], w2 = Ly.W2[o];
+          bf16x8 w1, w2, w3, wa1, wa2;
+          if constexpr (WS == WSRC_FP32) {
+            const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb + kb) * 64 + lane;
+            split_w8(P[0], P[64], w1, w2, w3, wa1, wa2);
+          } else {
+            w1 = Ly.W1[o]; w2 = Ly.W2[o]; w3 = Ly.W3[o];
+          }
           // base row, smallest products first, fresh accumulator per chunk
           {
-            const bf16x8 w3 = Ly.W3[o];
             floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x3, z4, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, x2, c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, x1, c, 0, 0, 0);
@@ -224,7 +272,12 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a1, c, 0, 0, 0);
             acc[f][1 + i][t] += c;
           }
-          acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ly.Wa[o], xe, z4, 0, 0, 0);
+          if constexpr (WS == WSRC_FP32) {
+            floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xe, z4, 0, 0, 0);
+            acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xe, c, 0, 0, 0);
+          } else {
+            acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ly.Wa[o], xe, z4, 0, 0, 0);
+          }
         }
       }
     }
@@ -248,7 +301,7 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
           v[0] = acc[f][0][t][j] + bias4[j];
 #pragma unroll
           for (int i = 0; i < V; ++i) v[1 + i] = acc[f][1 + i][t][j];
-          v[V + 1] = up1(acc[f][NS][t][j] * Ly.e_fix);
+          v[V + 1] = up1(acc[f][NS][t][j] * (WS == WSRC_FP32 ? Ly.e_fix32 : Ly.e_fix));
           if (last) {
             if (ft == 0 && lg == 0 && j == 0) fin(v, b);
             continue;
@@ -302,7 +355,7 @@ __device__ __forceinline__ void final_bounds_fp64(const float* v, double& lo, do
   scale = ::fabs((double)v[0]) + rad;
 }
 
-template <int V, int NB, int NW, int WMAX>
+template <int V, int NB, int NW, int WMAX, int WS>
 __global__ void __launch_bounds__(NW * 64)
 k_classify_bf16(const DevModelBf16* __restrict__ Mp, ClassifyArgs a) {
   using PB = PlanesBf16<V>;
@@ -341,7 +394,7 @@ k_classify_bf16(const DevModelBf16* __restrict__ Mp, ClassifyArgs a) {
       }
       store_split<V, NB, WMAX>(Xs, b, k, s, err, cb0, ca0);
     }
-    mlp_forward_bf16<V, NB, NW, WMAX>(*Mp, Xs, [&](const float* v, int b) {
+    mlp_forward_bf16<V, NB, NW, WMAX, WS>(*Mp, Xs, [&](const float* v, int b) {
       const long long i = i0 + b;
       if (i >= n) return;
       double lo, hi, scale;

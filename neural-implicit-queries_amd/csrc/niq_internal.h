@@ -109,6 +109,7 @@ struct niq_model {
   int in_dim = 0;
   int wt = 0;                  // template width
   int wt16 = 0;                // template width of the precision-mode copy
+  bool bf16_ok = true;         // no FP32 subnormal weights (precision mode)
   DevModelBf16 host16;         // precision-mode copy (bf16 hi/lo/|W| weights)
   DevModelBf16* dev16 = nullptr;
   void* weights16 = nullptr;
@@ -134,6 +135,12 @@ niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nv
 // ... with the optional FP64 re-test list in stream-ordered scratch.
 niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, int nvec, ClassifyArgs a,
                                  long long n, hipStream_t s);
+// niq_tree_enqueue with the root box optionally read from the device (6
+// floats, lower then upper), so that one captured build serves any root box.
+// Defined in api_tree.hip.
+niq_status tree_enqueue_async(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, void* ws,
+                              size_t ws_bytes, niq_tree_result* res, size_t* counters_offset, const float* root_dev,
+                              void* stream);
 
 }  // namespace niqi
 

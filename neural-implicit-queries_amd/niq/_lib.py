@@ -25,8 +25,11 @@ PREC = {"fp32": 0, "bf16": 1}
 
 EXPORTS = ("niq_model_create", "niq_model_destroy", "niq_model_device_bytes",
            "niq_classify_boxes", "niq_classify_general_boxes", "niq_eval_points",
-           "niq_tree_workspace_bytes", "niq_tree_build", "niq_cast_workspace_bytes",
+           "niq_tree_workspace_bytes", "niq_tree_build", "niq_tree_enqueue",
+           "niq_tree_counters_bytes", "niq_tree_decode", "niq_cast_workspace_bytes",
            "niq_cast_rays", "niq_closest_point", "niq_mc_extract", "niq_mc_table",
+           "niq_serve_create", "niq_serve_destroy", "niq_serve_submit_rays", "niq_serve_submit_tree",
+           "niq_serve_submit_closest_point", "niq_serve_poll", "niq_serve_bytes_in_use",
            "niq_last_error", "niq_version")
 
 
@@ -43,7 +46,8 @@ class CapacityError(NiqError):
 class EvalCfg(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("precision", ctypes.c_int),
                 ("truncate_count", ctypes.c_int), ("n_append", ctypes.c_int),
-                ("recheck_tol", ctypes.c_float), ("reserved", ctypes.c_int * 3)]
+                ("recheck_tol", ctypes.c_float), ("variant", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
 
 
 class TreeParams(ctypes.Structure):
@@ -74,6 +78,13 @@ class CastOpts(ctypes.Structure):
                 ("interval_shrink_fac", ctypes.c_float), ("init_step", ctypes.c_float)]
 
 
+class ServeInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("n_evals", ctypes.c_int64), ("n_batched", ctypes.c_int32),
+                ("graph_replay", ctypes.c_int32), ("n_processed", ctypes.c_int64)]
+
+
 _lib = None
 
 
@@ -95,11 +106,22 @@ def load_library():
         "niq_eval_points": (I, [P, P, I64, I, P, P]),
         "niq_tree_workspace_bytes": (I, [P, P, P, ctypes.POINTER(S)]),
         "niq_tree_build": (I, [P, P, P, P, S, P, P]),
+        "niq_tree_enqueue": (I, [P, P, P, P, S, P, ctypes.POINTER(S), P]),
+        "niq_tree_counters_bytes": (S, []),
+        "niq_tree_decode": (I, [P, P, P]),
         "niq_cast_workspace_bytes": (S, [P]),
         "niq_cast_rays": (I, [P, I, P, P, P, I64, P, P, P, P, P, S, ctypes.POINTER(I64), P]),
         "niq_closest_point": (I, [P, P, P, P, P, I64, F, I, P, P, ctypes.POINTER(I64), P]),
         "niq_mc_extract": (I, [P, P, P, I64, I, P, I64, ctypes.POINTER(I64), P]),
         "niq_mc_table": (I, [P]),
+        "niq_serve_create": (I, [I, S, I64, ctypes.POINTER(P)]),
+        "niq_serve_destroy": (I, [P]),
+        "niq_serve_submit_rays": (I, [P, P, I, P, P, P, I64, P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]),
+        "niq_serve_submit_tree": (I, [P, P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]),
+        "niq_serve_submit_closest_point": (I, [P, P, P, P, P, P, I64, F, I, P, P,
+                                               ctypes.POINTER(ctypes.c_uint64)]),
+        "niq_serve_poll": (I, [P, ctypes.c_uint64, I, P, P]),
+        "niq_serve_bytes_in_use": (S, [P]),
         "niq_last_error": (ctypes.c_char_p, []),
         "niq_version": (I, []),
     }
@@ -133,8 +155,8 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
-def eval_cfg(mode="affine_fixed", precision="fp32", truncate_count=0, n_append=0, recheck_tol=0.0):
+def eval_cfg(mode="affine_fixed", precision="fp32", truncate_count=0, n_append=0, recheck_tol=0.0, variant=0):
     if mode not in MODE:
         raise ValueError("invalid mode")
     return EvalCfg(MODE[mode], PREC[precision], int(truncate_count), int(n_append),
-                   float(recheck_tol), (ctypes.c_int * 3)())
+                   float(recheck_tol), int(variant), (ctypes.c_int * 2)())

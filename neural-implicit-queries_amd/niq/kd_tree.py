@@ -37,17 +37,16 @@ def _pad_rows(src, n, size):
     return out
 
 
-def build_tree(func, params, lower, upper, node_terminate_thresh=None, split_depth=None,
-               with_interior_nodes=False, with_exterior_nodes=False, offset=0.,
-               batch_process_size=2048, stream=None):
-    """Device tree build; returns (out_dict, stats)."""
+def tree_request(func, params, lower, upper, node_terminate_thresh=None, split_depth=None,
+                 with_interior_nodes=False, with_exterior_nodes=False, offset=0., batch_process_size=2048):
+    """Validate a tree query (src/kd_tree.py:105-111) and lower it to the ABI:
+    returns (device model, TreeParams, level capacity, finished-list capacity)."""
     for b in bucket_sizes:
         if b > batch_process_size and (b // batch_process_size) * batch_process_size != b:
             raise ValueError(f"batch_process_size must be a factor of our bucket sizes, is not a factor of {b} (try a power of 2)")
     if node_terminate_thresh is None and split_depth is None:
         raise ValueError("must specify at least one of node_terminate_thresh or split_depth as a terminating condition")
     _lib.require_gpu()
-    lib = _lib.load_library()
     m = _mlp.device_model(params)
     lower = [float(v) for v in torch.as_tensor(lower).reshape(-1).tolist()]
     upper = [float(v) for v in torch.as_tensor(upper).reshape(-1).tolist()]
@@ -63,29 +62,80 @@ def build_tree(func, params, lower, upper, node_terminate_thresh=None, split_dep
     p.batch_process_size = int(batch_process_size)
     p.with_interior_nodes = int(bool(with_interior_nodes))
     p.with_exterior_nodes = int(bool(with_exterior_nodes))
-    ws_bytes = ctypes.c_size_t()
-    _lib.check(lib.niq_tree_workspace_bytes(m.handle, ctypes.byref(func.cfg), ctypes.byref(p),
-                                           ctypes.byref(ws_bytes)))
-    dev = torch.device("cuda")
-    ws = torch.empty(ws_bytes.value, dtype=torch.uint8, device=dev)
     # level capacity = max nodes of any level (the workspace holds two level lists)
     cap = 1 << p.split_depth if p.split_depth >= 0 else 2 * p.node_terminate_thresh
     if p.node_terminate_thresh > 0:
         cap = min(cap, 2 * p.node_terminate_thresh)
     cap_fin = min(2 * cap, 1 << 26)
+    return m, p, cap, cap_fin
+
+
+class TreeBuffers:
+    """Caller-owned output buffers of one tree build and their ABI view."""
+
+    def __init__(self, p, cap, cap_fin, dev):
+        self.p = p
+        self.unk = torch.empty((2, cap, 3), dtype=torch.float32, device=dev)
+        self.fin_i = torch.empty((2, cap_fin, 3), dtype=torch.float32, device=dev) if p.with_interior_nodes else None
+        self.fin_e = torch.empty((2, cap_fin, 3), dtype=torch.float32, device=dev) if p.with_exterior_nodes else None
+        r = _lib.TreeResult()
+        r.unknown_lower, r.unknown_upper, r.unknown_capacity = self.unk[0].data_ptr(), self.unk[1].data_ptr(), cap
+        if self.fin_i is not None:
+            r.interior_lower, r.interior_upper = self.fin_i[0].data_ptr(), self.fin_i[1].data_ptr()
+            r.interior_capacity = cap_fin
+        if self.fin_e is not None:
+            r.exterior_lower, r.exterior_upper = self.fin_e[0].data_ptr(), self.fin_e[1].data_ptr()
+            r.exterior_capacity = cap_fin
+        self.r = r
+
+    def outputs(self):
+        """The reference's return dict (src/kd_tree.py:202-218) and build stats."""
+        r, p = self.r, self.p
+        dev = self.unk.device
+        n_lv = r.n_levels
+        level_n = list(r.level_n[:n_lv])
+        B = int(p.batch_process_size)
+        n_unk = int(r.n_unknown)
+        bucket = get_next_bucket_size(n_unk)
+        out = {
+            'unknown_node_valid': torch.arange(bucket, device=dev) < n_unk,
+            'unknown_node_lower': _pad_rows(self.unk[0], n_unk, bucket),
+            'unknown_node_upper': _pad_rows(self.unk[1], n_unk, bucket),
+        }
+        if self.fin_i is not None:
+            n = int(r.n_interior)
+            size = _finished_size(level_n, list(r.level_interior[:n_lv]), B)
+            out['interior_node_valid'] = torch.arange(size, device=dev) < n
+            out['interior_node_lower'] = _pad_rows(self.fin_i[0], n, size)
+            out['interior_node_upper'] = _pad_rows(self.fin_i[1], n, size)
+        if self.fin_e is not None:
+            n = int(r.n_exterior)
+            size = _finished_size(level_n, list(r.level_exterior[:n_lv]), B)
+            out['exterior_node_valid'] = torch.arange(size, device=dev) < n
+            out['exterior_node_lower'] = _pad_rows(self.fin_e[0], n, size)
+            out['exterior_node_upper'] = _pad_rows(self.fin_e[1], n, size)
+        stats = {"n_classified": int(r.n_classified), "n_unknown": n_unk,
+                 "n_interior": int(r.n_interior), "n_exterior": int(r.n_exterior),
+                 "n_recheck": int(r.n_recheck), "level_counts": level_n}
+        return out, stats
+
+
+def build_tree(func, params, lower, upper, node_terminate_thresh=None, split_depth=None,
+               with_interior_nodes=False, with_exterior_nodes=False, offset=0.,
+               batch_process_size=2048, stream=None):
+    """Device tree build; returns (out_dict, stats)."""
+    m, p, cap, cap_fin = tree_request(func, params, lower, upper, node_terminate_thresh, split_depth,
+                                      with_interior_nodes, with_exterior_nodes, offset, batch_process_size)
+    lib = _lib.load_library()
+    ws_bytes = ctypes.c_size_t()
+    _lib.check(lib.niq_tree_workspace_bytes(m.handle, ctypes.byref(func.cfg), ctypes.byref(p),
+                                           ctypes.byref(ws_bytes)))
+    dev = torch.device("cuda")
+    ws = torch.empty(ws_bytes.value, dtype=torch.uint8, device=dev)
     s = _lib.stream_handle(stream)
     while True:
-        unk = torch.empty((2, cap, 3), dtype=torch.float32, device=dev)
-        fin_i = torch.empty((2, cap_fin, 3), dtype=torch.float32, device=dev) if with_interior_nodes else None
-        fin_e = torch.empty((2, cap_fin, 3), dtype=torch.float32, device=dev) if with_exterior_nodes else None
-        r = _lib.TreeResult()
-        r.unknown_lower, r.unknown_upper, r.unknown_capacity = unk[0].data_ptr(), unk[1].data_ptr(), cap
-        if fin_i is not None:
-            r.interior_lower, r.interior_upper = fin_i[0].data_ptr(), fin_i[1].data_ptr()
-            r.interior_capacity = cap_fin
-        if fin_e is not None:
-            r.exterior_lower, r.exterior_upper = fin_e[0].data_ptr(), fin_e[1].data_ptr()
-            r.exterior_capacity = cap_fin
+        bufs = TreeBuffers(p, cap, cap_fin, dev)
+        r = bufs.r
         st = lib.niq_tree_build(m.handle, ctypes.byref(func.cfg), ctypes.byref(p), _lib.ptr(ws),
                                 ws_bytes.value, ctypes.byref(r), s)
         if st == _lib.NIQ_ERR_CAPACITY and max(r.n_interior, r.n_exterior) > cap_fin:
@@ -93,32 +143,7 @@ def build_tree(func, params, lower, upper, node_terminate_thresh=None, split_dep
             continue
         _lib.check(st)
         break
-    n_lv = r.n_levels
-    level_n = list(r.level_n[:n_lv])
-    B = int(batch_process_size)
-    n_unk = int(r.n_unknown)
-    bucket = get_next_bucket_size(n_unk)
-    out = {
-        'unknown_node_valid': torch.arange(bucket, device=dev) < n_unk,
-        'unknown_node_lower': _pad_rows(unk[0], n_unk, bucket),
-        'unknown_node_upper': _pad_rows(unk[1], n_unk, bucket),
-    }
-    if with_interior_nodes:
-        n = int(r.n_interior)
-        size = _finished_size(level_n, list(r.level_interior[:n_lv]), B)
-        out['interior_node_valid'] = torch.arange(size, device=dev) < n
-        out['interior_node_lower'] = _pad_rows(fin_i[0], n, size)
-        out['interior_node_upper'] = _pad_rows(fin_i[1], n, size)
-    if with_exterior_nodes:
-        n = int(r.n_exterior)
-        size = _finished_size(level_n, list(r.level_exterior[:n_lv]), B)
-        out['exterior_node_valid'] = torch.arange(size, device=dev) < n
-        out['exterior_node_lower'] = _pad_rows(fin_e[0], n, size)
-        out['exterior_node_upper'] = _pad_rows(fin_e[1], n, size)
-    stats = {"n_classified": int(r.n_classified), "n_unknown": n_unk,
-             "n_interior": int(r.n_interior), "n_exterior": int(r.n_exterior),
-             "n_recheck": int(r.n_recheck), "level_counts": level_n}
-    return out, stats
+    return bufs.outputs()
 
 
 def construct_uniform_unknown_levelset_tree(func, params, lower, upper, node_terminate_thresh=None,

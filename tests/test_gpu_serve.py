@@ -1,0 +1,152 @@
+"""Serving loop: each result is the same alone and in the mix.
+
+Bar: identical outputs, bit for bit.  A served request runs the same kernels
+on the same inputs as the standalone call: rays are independent of one
+another (a merged cast gives every ray what a lone cast gives it), and each
+tree build and closest-point batch runs as one call of its own.  Checked with
+requests merged (one worker kept busy so that ray requests queue), with three
+workers overlapping on the device, with tree builds replayed from a captured
+hipGraph over different root boxes, and with memory admission serialising
+requests that do not fit together.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from niq import implicit_mlp_utils, kd_tree, queries, render, models, serve
+from niq import _lib
+
+pytestmark = pytest.mark.gpu
+
+LO = np.array([-1., -1., -1.], np.float32)
+UP = np.array([1., 1., 1.], np.float32)
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def camera(res, eye=(0., 0., 3.)):
+    eye = np.array(eye, np.float32)
+    look, up, _ = render.look_at(eye)
+    return render.generate_camera_rays(eye, look, up, res=res, fov_deg=60.0, device=None)
+
+
+def setup():
+    p1 = models.config_model("c1_relu_3x32")
+    p2 = models.random_mlp([48, 40, 24], activation="tanh", seed=3)
+    f1, _ = implicit_mlp_utils.implicit_from_params(p1, "affine_fixed")
+    f2, _ = implicit_mlp_utils.implicit_from_params(p2, "affine_fixed")
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    return (f1, p1), (f2, p2), opts
+
+
+def tiles(roots, dirs, sizes):
+    out, o = [], 0
+    for s in sizes:
+        out.append((roots[o:o + s], dirs[o:o + s]))
+        o += s
+    return out
+
+
+def same_rays(a, b):
+    return all(torch.equal(x, y) for x, y in zip(a[:3], b[:3]))
+
+
+def same_tree(a, b):
+    return a.keys() == b.keys() and all(torch.equal(a[k], b[k]) for k in a)
+
+
+def test_merged_rays_equal_lone_casts():
+    note("[serve] merged ray casts")
+    (f1, p1), (f2, p2), opts = setup()
+    roots, dirs = camera(80)
+    chunks = tiles(roots, dirs, [700, 1900, 64, 2000, 1736])
+    alone = [queries.cast_rays((f1,), (p1,), r, d, opts) for r, d in chunks]
+    with serve.Server(n_streams=1) as srv:
+        # the tree keeps the only worker busy, so the casts queue and merge
+        busy = srv.submit_tree(f2, p2, LO, UP, split_depth=16)
+        reqs = [srv.submit_rays((f1,), (p1,), r, d, opts) for r, d in chunks]
+        busy.result()
+        got = [r.result() for r in reqs]
+        batched = [r.info["n_batched"] for r in reqs]
+    note(f"  merged group sizes {batched}")
+    assert max(batched) > 1
+    for a, g in zip(alone, got):
+        assert same_rays(a, g)
+
+
+def test_mix_equals_standalone():
+    note("[serve] mixed requests on three streams")
+    (f1, p1), (f2, p2), opts = setup()
+    roots, dirs = camera(64)
+    chunks = tiles(roots, dirs, [1024, 3072])
+    boxes = [(LO, UP), (np.array([-1, -1, -1], np.float32), np.array([0, 0, 0], np.float32)),
+             (np.array([-0.5, -0.25, -1], np.float32), np.array([0.5, 0.75, 0], np.float32))]
+    q = np.random.default_rng(5).uniform(-1, 1, size=(48, 3)).astype(np.float32)
+
+    alone_rays = [queries.cast_rays((f, ), (p, ), r, d, opts) for (f, p) in ((f1, p1), (f2, p2)) for r, d in chunks]
+    alone_trees = [kd_tree.construct_uniform_unknown_levelset_tree(f1, p1, lo, up, split_depth=10,
+                                                                   with_interior_nodes=True,
+                                                                   with_exterior_nodes=True)
+                   for lo, up in boxes]
+    alone_trees.append(kd_tree.construct_uniform_unknown_levelset_tree(f2, p2, LO, UP, split_depth=12))
+    alone_cp = kd_tree.closest_point(f2, p2, LO, UP, q, eps=0.05)
+
+    with serve.Server(n_streams=3) as srv:
+        t_rays, t_trees = [], []
+        # interleaved arrivals; each tree shape is submitted more than once so
+        # that later builds replay the captured graph
+        for rep in range(2):
+            for i, (lo, up) in enumerate(boxes):
+                t_trees.append((i, srv.submit_tree(f1, p1, lo, up, split_depth=10, with_interior_nodes=True,
+                                                   with_exterior_nodes=True)))
+                if i < len(chunks):
+                    for k, (f, p) in enumerate(((f1, p1), (f2, p2))):
+                        r, d = chunks[i]
+                        t_rays.append((k * len(chunks) + i, srv.submit_rays((f,), (p,), r, d, opts)))
+            t_trees.append((3, srv.submit_tree(f2, p2, LO, UP, split_depth=12)))
+        cp = srv.submit_closest_point(f2, p2, LO, UP, q, eps=0.05)
+        for i, r in t_rays:
+            assert same_rays(alone_rays[i], r.result()), f"ray request {i}"
+        replays = 0
+        for i, r in t_trees:
+            assert same_tree(alone_trees[i], r.result()), f"tree request {i}"
+            replays += r.info["graph_replay"]
+        d, loc = cp.result()
+        assert torch.equal(d, alone_cp[0]) and torch.equal(loc, alone_cp[1])
+        lat = serve.latency_summary([r for _, r in t_rays] + [r for _, r in t_trees] + [cp])
+        assert srv.bytes_in_use() > 0          # cached graphs stay resident
+    note(f"  graph replays {replays} of {len(t_trees)} trees; latency {lat}")
+    assert replays == len(t_trees)
+
+
+def test_memory_admission_serialises():
+    note("[serve] memory admission")
+    (f1, p1), _, _ = setup()
+    alone = kd_tree.construct_uniform_unknown_levelset_tree(f1, p1, LO, UP, split_depth=16)
+    lib = _lib.load_library()
+    m, p, cap, cap_fin = kd_tree.tree_request(f1, p1, LO, UP, split_depth=16)
+    import ctypes
+    need = ctypes.c_size_t()
+    _lib.check(lib.niq_tree_workspace_bytes(m.handle, ctypes.byref(f1.cfg), ctypes.byref(p), ctypes.byref(need)))
+    # room for one build at a time (graphs take 1/8 of the cap: too little here)
+    with serve.Server(n_streams=2, mem_cap_bytes=int(need.value * 1.5)) as srv:
+        t0 = time.perf_counter()
+        a = srv.submit_tree(f1, p1, LO, UP, split_depth=16)
+        t1 = time.perf_counter()
+        b = srv.submit_tree(f1, p1, LO, UP, split_depth=16)
+        ra, rb = a.result(), b.result()
+        assert not a.info["graph_replay"] and not b.info["graph_replay"]
+        done_a = (t0 - t0) * 1e3 + a.info["latency_ms"]
+        start_b = (t1 - t0) * 1e3 + b.info["queue_ms"]
+        note(f"  a done at {done_a:.3f} ms, b started at {start_b:.3f} ms")
+        assert start_b >= done_a - 0.5
+        with pytest.raises(_lib.CapacityError):
+            srv.submit_tree(f1, p1, LO, UP, split_depth=18)
+    assert same_tree(alone, ra) and same_tree(alone, rb)
