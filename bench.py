@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 vs FP32 comparison")
     ap.add_argument("--precision-depth", type=int, default=21)
+    ap.add_argument("--serve-requests", type=int, default=120,
+                    help="requests in the serving trace (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
     return ap.parse_args()
 
 
@@ -88,6 +91,84 @@ def precision_compare(depth):
                      "tflops": models.flops_per_box(dims, 5) * st["n_classified"] / dt / 1e12}
     res["speedup_bf16_over_fp32"] = res["fp32"]["seconds"] / res["bf16"]["seconds"]
     return res
+
+
+SERVE_SHAPES = (("c1 3->32x3->1 relu", [32] * 3, "relu", "he"),
+                ("c2 3->256x6->1 relu", [256] * 6, "relu", "he"),
+                ("c3 3->128x8->1 tanh", [128] * 8, "tanh", "xavier"))
+
+
+def serving_trace(n_req, rate, seed=0):
+    """A reduced serving trace: eight resident MLPs (seeds 10-17, shapes drawn
+    from the C1-C3 shapes) under affine_fixed; Poisson arrivals (seed 0) of
+    50 % ray-cast tiles (side U{64..512} of the 60-degree camera at distance 3),
+    30 % closest-point batches (U{1..512} points in [-1,1]^3, eps 0.01) and
+    20 % level-set trees (split depth U{12..21}); memory cap 200 GB.  Reports
+    p50/p99 latency per kind and overall, and requests/s."""
+    import torch
+    from niq import models, implicit_mlp_utils, queries, render, serve
+    rng = np.random.default_rng(seed)
+    resident = []
+    for s_ in range(10, 18):
+        name, hidden, act, init = SERVE_SHAPES[int(rng.integers(0, 3))]
+        p = models.random_mlp(hidden, activation=act, init=init, seed=s_)
+        f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+        resident.append((name, f, p))
+    eye = np.array([0., 0., 3.], np.float32)
+    look, upd, _ = render.look_at(eye)
+    cams = {}
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    plan = []
+    for i in range(n_req):
+        u = rng.uniform()
+        k = int(rng.integers(0, len(resident)))
+        if u < 0.5:
+            side = int(rng.integers(64, 513))
+            plan.append(("rays", k, side))
+        elif u < 0.8:
+            plan.append(("closest_point", k, int(rng.integers(1, 513))))
+        else:
+            plan.append(("tree", k, int(rng.integers(12, 22))))
+    # inputs built before the clock starts (resident in HBM)
+    inputs = []
+    for kind, k, arg in plan:
+        if kind == "rays":
+            if arg not in cams:
+                cams[arg] = render.generate_camera_rays(eye, look, upd, res=arg, fov_deg=60.)
+            inputs.append(cams[arg])
+        elif kind == "closest_point":
+            inputs.append(torch.as_tensor(rng.uniform(-1, 1, size=(arg, 3)).astype(np.float32), device="cuda"))
+        else:
+            inputs.append(None)
+    torch.cuda.synchronize()
+    reqs = []
+    with serve.Server(n_streams=4, mem_cap_bytes=200 * 10 ** 9) as srv:
+        t0 = time.perf_counter()
+        for (kind, k, arg), inp, at in zip(plan, inputs, arrivals):
+            wait = at - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            _, f, p = resident[k]
+            if kind == "rays":
+                reqs.append(srv.submit_rays((f,), (p,), inp[0], inp[1], opts))
+            elif kind == "closest_point":
+                reqs.append(srv.submit_closest_point(f, p, lo, up, inp, eps=0.01))
+            else:
+                reqs.append(srv.submit_tree(f, p, lo, up, split_depth=arg))
+        for r in reqs:
+            r.result()
+        wall = time.perf_counter() - t0
+    lat = serve.latency_summary(reqs)
+    merged = sum(1 for r in reqs if r.kind == "rays" and r.info["n_batched"] > 1)
+    replays = sum(1 for r in reqs if r.kind == "tree" and r.info["graph_replay"])
+    return {"workload": f"{n_req} requests, Poisson {rate:g}/s (seed {seed}), 8 resident MLPs, "
+                        "50% ray tiles 64-512 px, 30% closest-point 1-512 pts, 20% trees depth 12-21",
+            "requests_per_s": n_req / wall, "wall_s": wall, "latency_ms": lat,
+            "rays_merged_requests": merged, "trees_from_graphs": replays,
+            "models": [m[0] for m in resident]}
 
 
 def main():
@@ -183,6 +264,8 @@ def main():
     line["latency_ms"] = {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)}
     if not args.no_precision:
         line["precision"] = precision_compare(args.precision_depth)
+    if args.serve_requests > 0:
+        line["serving"] = serving_trace(args.serve_requests, args.serve_rate)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(params, args.cpu_seconds)
     print(json.dumps(line), flush=True)

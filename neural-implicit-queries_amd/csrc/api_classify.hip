@@ -1,15 +1,13 @@
 // Per-box classification and point evaluation entry points.
 #include "niq_internal.h"
-#include "niq_bf16.h"
-#include "niq_rows.h"
 
 // ------------------------------------------------------------- dispatch
 namespace {
 
-// Launch k_classify for (KIND, V) at template width W.
-template <int KIND, int V, int W>
+// Launch k_classify for (KIND, V) at template width W with tile T.
+template <int KIND, int V, int W, class T = TileCfg<W>>
 niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
-  constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
+  constexpr int NB = T::NB, NW = T::NW;
   constexpr int R = Planes<KIND, V, 0>::R;
   if constexpr (!Fits<R, NB, W>::value) {
     return fail(NIQ_ERR_UNSUPPORTED, "range form does not fit in LDS at this width");
@@ -30,99 +28,28 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
   }
 }
 
+// Tile variants: variant 2 selects TileCfgAlt for the forms trees use most
+// (interval and affine_fixed over 3 box vectors); otherwise TileCfg.
 template <int KIND, int V>
-niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s,
+                           int variant = 0) {
+  constexpr bool has_alt = KIND == 0 && (V == 0 || V == 3);
+  if constexpr (has_alt) {
+    if (variant == 2) {
+      switch (m->wt) {
+        case 64: return launch_classify_w<KIND, V, 64, TileCfgAlt<64>>(m, a, max_n, s);
+        case 128: return launch_classify_w<KIND, V, 128, TileCfgAlt<128>>(m, a, max_n, s);
+        case 256: return launch_classify_w<KIND, V, 256, TileCfgAlt<256>>(m, a, max_n, s);
+        case 512: return launch_classify_w<KIND, V, 512, TileCfgAlt<512>>(m, a, max_n, s);
+      }
+      return fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+  }
   switch (m->wt) {
     case 64: return launch_classify_w<KIND, V, 64>(m, a, max_n, s);
     case 128: return launch_classify_w<KIND, V, 128>(m, a, max_n, s);
     case 256: return launch_classify_w<KIND, V, 256>(m, a, max_n, s);
     case 512: return launch_classify_w<KIND, V, 512>(m, a, max_n, s);
-  }
-  return fail(NIQ_ERR_UNSUPPORTED, "width");
-}
-
-template <int V, int W, int WS>
-niq_status launch_bf16_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
-  constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
-  if constexpr ((size_t)PlanesBf16<V>::NPL * NB * W * 2 > (size_t)kLdsLimit) {
-    return fail(NIQ_ERR_UNSUPPORTED, "precision-mode range form does not fit in LDS at this width");
-  } else {
-    auto k = k_classify_bf16<V, NB, NW, W, WS>;
-    const long long tiles = (max_n + NB - 1) / NB;
-    if (tiles == 0) return NIQ_OK;
-    const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
-    hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev16, a);
-    HIPCHK(hipGetLastError());
-    return NIQ_OK;
-  }
-}
-
-template <int V, int WS>
-niq_status launch_bf16_ws(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
-  switch (m->wt16) {
-    case 64: return launch_bf16_w<V, 64, WS>(m, a, max_n, s);
-    case 128: return launch_bf16_w<V, 128, WS>(m, a, max_n, s);
-    case 256: return launch_bf16_w<V, 256, WS>(m, a, max_n, s);
-    case 512: return launch_bf16_w<V, 512, WS>(m, a, max_n, s);
-  }
-  return fail(NIQ_ERR_UNSUPPORTED, "width");
-}
-
-// Weight source: cfg->variant = 1 streams the four packed bf16 copies, 2 (and
-// the default) the FP32 weights split in registers.
-template <int V>
-niq_status launch_bf16(const niq_model* m, const niq_eval_cfg* cfg, const ClassifyArgs& a, long long max_n,
-                       hipStream_t s) {
-  if (!m->bf16_ok) return fail(NIQ_ERR_UNSUPPORTED, "the precision mode needs weights without FP32 subnormals");
-  const int ws = cfg ? cfg->variant : 0;
-  if (ws == 1) return launch_bf16_ws<V, WSRC_BF16>(m, a, max_n, s);
-  return launch_bf16_ws<V, WSRC_FP32>(m, a, max_n, s);
-}
-
-// affine_all / affine_truncate / affine_append: one box per workgroup, error
-// terms streamed through a private HBM scratch.
-template <int W, int MODE>
-niq_status launch_rows_w(const niq_model* m, const niq_eval_cfg* cfg, const ClassifyArgs& a, long long max_n,
-                         hipStream_t s) {
-  constexpr int NW = RowsCfg<W>::NW;
-  auto k = k_classify_rows<W, MODE>;
-  const long long d0 = std::max(a.d, a.lower ? 0 : a.vin);
-  const int n_keep = cfg ? cfg->truncate_count : 0;
-  const int n_append = cfg ? cfg->n_append : 0;
-  long long max_rows = d0;
-  int n_act = 0;
-  for (int i = 0; i + 1 < m->n_layers; ++i)
-    if (m->acts[i] != NIQ_ACT_NONE) { max_rows += m->host.L[i].out_pad; ++n_act; }
-  if (MODE == ROWS_APPEND) max_rows = d0 + (long long)n_act * n_append;
-  if (MODE == ROWS_TRUNCATE) max_rows = d0 + n_keep + 2LL * W;
-  max_rows = std::max<long long>(max_rows, 1);
-  const size_t per_wg = sizeof(float) * 2 * (size_t)max_rows * W;
-  if (max_n == 0) return NIQ_OK;
-  long long g = std::min<long long>(max_n, persistent_blocks(k, NW * 64));
-  g = std::max<long long>(1, std::min<long long>(g, (long long)((16ull << 30) / per_wg)));
-  void* scratch = nullptr;
-  HIPCHK(hipMallocAsync(&scratch, per_wg * g, s));
-  RowsArgs ra{};
-  ra.c = a;
-  ra.scratch = static_cast<float*>(scratch);
-  ra.max_rows = max_rows;
-  ra.n_keep = n_keep;
-  ra.n_append = n_append;
-  hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(NW * 64), 0, s, m->dev, ra);
-  const hipError_t e = hipGetLastError();
-  HIPCHK(hipFreeAsync(scratch, s));
-  if (e != hipSuccess) return fail(NIQ_ERR_HIP, std::string("k_classify_rows: ") + hipGetErrorString(e));
-  return NIQ_OK;
-}
-
-template <int MODE>
-niq_status launch_rows(const niq_model* m, const niq_eval_cfg* cfg, const ClassifyArgs& a, long long max_n,
-                       hipStream_t s) {
-  switch (m->wt) {
-    case 64: return launch_rows_w<64, MODE>(m, cfg, a, max_n, s);
-    case 128: return launch_rows_w<128, MODE>(m, cfg, a, max_n, s);
-    case 256: return launch_rows_w<256, MODE>(m, cfg, a, max_n, s);
-    case 512: return launch_rows_w<512, MODE>(m, cfg, a, max_n, s);
   }
   return fail(NIQ_ERR_UNSUPPORTED, "width");
 }
@@ -134,39 +61,20 @@ niq_status niqi::classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, 
                                    const ClassifyArgs& a, long long max_n, hipStream_t s) {
   const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
   const int prec = cfg ? cfg->precision : NIQ_PREC_FP32;
-  if (prec == NIQ_PREC_BF16) {
-    ClassifyArgs b = a;
-    b.recheck_tol = 0.f;   // the precision mode is rigorous by construction
-    if (mode == NIQ_MODE_INTERVAL) return launch_bf16<0>(m, cfg, b, max_n, s);
-    if (mode == NIQ_MODE_AFFINE_FIXED) {
-      switch (nvec) {
-        case 1: return launch_bf16<1>(m, cfg, b, max_n, s);
-        case 2: return launch_bf16<2>(m, cfg, b, max_n, s);
-        case 3: return launch_bf16<3>(m, cfg, b, max_n, s);
-      }
-    }
-    return fail(NIQ_ERR_UNSUPPORTED, "the precision mode supports interval and affine_fixed with 1..3 box vectors");
-  }
+  if (prec == NIQ_PREC_BF16) return classify_bf16(m, cfg, nvec, a, max_n, s);
   if (prec != NIQ_PREC_FP32) return fail(NIQ_ERR_INVALID, "unknown precision");
   if (mode == NIQ_MODE_AFFINE_ALL || mode == NIQ_MODE_AFFINE_TRUNCATE || mode == NIQ_MODE_AFFINE_APPEND) {
     if (a.recheck_tol > 0.f) return fail(NIQ_ERR_UNSUPPORTED, "the growing-row modes have no FP64 re-test");
     if (!a.lower && a.vin > 64) return fail(NIQ_ERR_UNSUPPORTED, "at most 64 box vectors");
-    if (mode == NIQ_MODE_AFFINE_ALL) return launch_rows<ROWS_ALL>(m, cfg, a, max_n, s);
-    if (mode == NIQ_MODE_AFFINE_TRUNCATE) {
-      if (!cfg || cfg->truncate_count < 1 || cfg->truncate_count > 64)
-        return fail(NIQ_ERR_INVALID, "affine_truncate needs 1 <= truncate_count <= 64");
-      return launch_rows<ROWS_TRUNCATE>(m, cfg, a, max_n, s);
-    }
-    if (!cfg || cfg->n_append < 0 || cfg->n_append > 512)
-      return fail(NIQ_ERR_INVALID, "affine_append needs 0 <= n_append <= 512");
-    return launch_rows<ROWS_APPEND>(m, cfg, a, max_n, s);
+    return classify_rows(m, cfg, a, max_n, s);
   }
-  if (mode == NIQ_MODE_INTERVAL) return launch_classify<0, 0>(m, a, max_n, s);
+  const int variant = cfg ? cfg->variant : 0;
+  if (mode == NIQ_MODE_INTERVAL) return launch_classify<0, 0>(m, a, max_n, s, variant);
   if (mode == NIQ_MODE_AFFINE_FIXED) {
     switch (nvec) {
       case 1: return launch_classify<0, 1>(m, a, max_n, s);
       case 2: return launch_classify<0, 2>(m, a, max_n, s);
-      case 3: return launch_classify<0, 3>(m, a, max_n, s);
+      case 3: return launch_classify<0, 3>(m, a, max_n, s, variant);
     }
     return fail(NIQ_ERR_UNSUPPORTED, "affine_fixed supports 1..3 box vectors");
   }

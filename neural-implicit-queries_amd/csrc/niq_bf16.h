@@ -232,6 +232,19 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
         for (int t = 0; t < BT; ++t) acc[f][p][t] = z4;
 
     __syncthreads();
+    // FP32 weight fragments are fetched one K chunk ahead
+    floatx4 wpf[FTMAX][2];
+    if constexpr (WS == WSRC_FP32) {
+#pragma unroll
+      for (int f = 0; f < FTMAX; ++f) {
+        const int ft = wave + f * NW;
+        if (ft < nft) {
+          const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb) * 64 + lane;
+          wpf[f][0] = P[0];
+          wpf[f][1] = P[64];
+        }
+      }
+    }
     for (int kb = 0; kb < nkb; ++kb) {
 #pragma unroll
       for (int t = 0; t < BT; ++t) {
@@ -248,8 +261,14 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
           const size_t o = (size_t)(ft * nkb + kb) * 64 + lane;
           bf16x8 w1, w2, w3, wa1, wa2;
           if constexpr (WS == WSRC_FP32) {
-            const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb + kb) * 64 + lane;
-            split_w8(P[0], P[64], w1, w2, w3, wa1, wa2);
+            // split this chunk's fragment, then fetch the next chunk's into
+            // the same registers
+            split_w8(wpf[f][0], wpf[f][1], w1, w2, w3, wa1, wa2);
+            if (t + 1 == BT && kb + 1 < nkb) {
+              const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb + kb + 1) * 64 + lane;
+              wpf[f][0] = P[0];
+              wpf[f][1] = P[64];
+            }
           } else {
             w1 = Ly.W1[o]; w2 = Ly.W2[o]; w3 = Ly.W3[o];
           }

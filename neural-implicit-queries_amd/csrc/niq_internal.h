@@ -132,6 +132,12 @@ inline niq_status check_model(const niq_model* m, int d) {
 // by (mode, precision, box vectors).  Defined in api_classify.hip.
 niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec, const ClassifyArgs& a,
                              long long max_n, hipStream_t s);
+// The precision-mode launches (api_classify_bf16.hip) and the growing-row
+// modes (api_classify_rows.hip).
+niq_status classify_bf16(const niq_model* m, const niq_eval_cfg* cfg, int nvec, const ClassifyArgs& a,
+                         long long max_n, hipStream_t s);
+niq_status classify_rows(const niq_model* m, const niq_eval_cfg* cfg, const ClassifyArgs& a, long long max_n,
+                         hipStream_t s);
 // ... with the optional FP64 re-test list in stream-ordered scratch.
 niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, int nvec, ClassifyArgs a,
                                  long long n, hipStream_t s);

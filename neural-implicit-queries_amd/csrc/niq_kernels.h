@@ -12,6 +12,15 @@ template <> struct TileCfg<128> { static constexpr int NB = 32, NW = 4, NBP = 64
 template <> struct TileCfg<256> { static constexpr int NB = 16, NW = 8, NBP = 64; };
 template <> struct TileCfg<512> { static constexpr int NB = 16, NW = 8, NBP = 64; };
 
+// Alternative tiles offered to the autotuner (eval cfg variant 2): twice the
+// boxes per workgroup where LDS allows it (weights read half as often per box),
+// fewer waves per workgroup at width 512.
+template <int WMAX> struct TileCfgAlt;
+template <> struct TileCfgAlt<64>  { static constexpr int NB = 64, NW = 4; };
+template <> struct TileCfgAlt<128> { static constexpr int NB = 64, NW = 8; };
+template <> struct TileCfgAlt<256> { static constexpr int NB = 32, NW = 8; };
+template <> struct TileCfgAlt<512> { static constexpr int NB = 16, NW = 4; };
+
 constexpr int kLdsLimit = 160 * 1024;
 
 template <int R, int NB, int WMAX>
