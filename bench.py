@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 vs FP32 comparison")
     ap.add_argument("--precision-depth", type=int, default=21)
+    ap.add_argument("--no-concurrent", action="store_true", help="skip the 64 concurrent trees")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests in the serving trace (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
@@ -171,6 +172,56 @@ def serving_trace(n_req, rate, seed=0):
             "models": [m[0] for m in resident]}
 
 
+def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):
+    """64 level-set trees over 16 resident TanH MLPs (Xavier init, seeds
+    20-35; widths from {32, 64, 128, 256} and 3-8 hidden layers drawn from
+    seed 1), affine_fixed with autotuned kernel variants, built one after
+    another and then all submitted at once to the serving loop (4 streams)."""
+    import torch
+    from niq import models, implicit_mlp_utils, kd_tree, serve, autotune
+    rng = np.random.default_rng(seed)
+    fams = []
+    for i in range(n_models):
+        w = int(rng.choice([32, 64, 128, 256]))
+        nl = int(rng.integers(3, 9))
+        p = models.random_mlp([w] * nl, activation="tanh", init="xavier", seed=20 + i)
+        f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+        autotune.tune(f, p, split_depth=depth, n_boxes=1 << 14, reps=2)
+        fams.append((f, p, w, nl))
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    for f, p, _, _ in fams:
+        kd_tree.build_tree(f, p, lo, up, split_depth=8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    boxes = 0
+    for i in range(n_trees):
+        f, p, _, _ = fams[i % n_models]
+        _, st = kd_tree.build_tree(f, p, lo, up, split_depth=depth)
+        boxes += st["n_classified"]
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    with serve.Server(n_streams=4) as srv:
+        # first pass captures each worker's graphs; the timed pass replays them
+        warm = [srv.submit_tree(fams[i % n_models][0], fams[i % n_models][1], lo, up, split_depth=depth)
+                for i in range(n_trees)]
+        for r in warm:
+            r.result()
+        t0 = time.perf_counter()
+        reqs = [srv.submit_tree(fams[i % n_models][0], fams[i % n_models][1], lo, up, split_depth=depth)
+                for i in range(n_trees)]
+        for r in reqs:
+            r.result()
+        t_srv = time.perf_counter() - t0
+    lat = serve.latency_summary(reqs)
+    return {"workload": f"{n_trees} trees, split depth {depth}, over {n_models} TanH MLPs "
+                        "(widths 32-256, 3-8 hidden layers), affine_fixed",
+            "boxes": boxes, "sequential_s": t_seq, "served_s": t_srv,
+            "boxes_per_s_sequential": boxes / t_seq, "boxes_per_s_served": boxes / t_srv,
+            "latency_ms": lat.get("tree"),
+            "variants": [int(f.cfg.variant) for f, _, _, _ in fams],
+            "shapes": [f"{w}x{nl}" for _, _, w, nl in fams]}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -264,6 +315,8 @@ def main():
     line["latency_ms"] = {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)}
     if not args.no_precision:
         line["precision"] = precision_compare(args.precision_depth)
+    if not args.no_concurrent:
+        line["concurrent_trees"] = concurrent_trees()
     if args.serve_requests > 0:
         line["serving"] = serving_trace(args.serve_requests, args.serve_rate)
     if not args.no_cpu:

@@ -9,10 +9,13 @@ namespace {
 struct TreeLayout {
   long long cap;      // max nodes in any level list
   long long nbmax;    // max batches per level
-  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], off_recheck, total;
+  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], off_recheck, off_lw, lw_bytes, total;
 };
 
-niq_status tree_layout(const niq_tree_params* p, TreeLayout* L) {
+// Boxes per chunk of the layer-by-layer precision mode inside a tree build.
+constexpr long long kTreeLwChunk = 32768;
+
+niq_status tree_layout(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, TreeLayout* L) {
   if (!p) return fail(NIQ_ERR_INVALID, "null params");
   const int B = p->batch_process_size;
   if (B < 1 || (B & (B - 1)) || B > (1 << 24))
@@ -38,6 +41,14 @@ niq_status tree_layout(const niq_tree_params* p, TreeLayout* L) {
   L->off_lvl[0] = take(sizeof(float) * 6 * cap);
   L->off_lvl[1] = take(sizeof(float) * 6 * cap);
   L->off_recheck = take(sizeof(int) * cap);
+  // activation scratch of the layer-by-layer precision mode (variant 3):
+  // two ping-pong buffers of 10 bf16 planes at the widest layer
+  L->lw_bytes = 0;
+  if (m && cfg && cfg->precision == NIQ_PREC_BF16 && cfg->variant == 3) {
+    const long long nch = std::min<long long>((cap + 63) / 64 * 64, kTreeLwChunk);
+    L->lw_bytes = 2 * (size_t)nch * 10 * (size_t)std::max(m->host16.wmax, 32) * 2;
+  }
+  L->off_lw = take(L->lw_bytes);
   L->total = o;
   return NIQ_OK;
 }
@@ -64,9 +75,8 @@ __global__ void k_tree_init(TreeCounters* C, float* lo, float* up, float l0, flo
 
 niq_status niq_tree_workspace_bytes(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p,
                                     size_t* bytes) {
-  (void)m; (void)cfg;
   TreeLayout L;
-  if (niq_status st = tree_layout(p, &L)) return st;
+  if (niq_status st = tree_layout(m, cfg, p, &L)) return st;
   if (bytes) *bytes = L.total;
   return NIQ_OK;
 }
@@ -99,7 +109,7 @@ niq_status tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg, const niq_t
   if (niq_status st = check_model(m, 3)) return st;
   if (!res) return fail(NIQ_ERR_INVALID, "null result");
   TreeLayout L;
-  if (niq_status st = tree_layout(p, &L)) return st;
+  if (niq_status st = tree_layout(m, cfg, p, &L)) return st;
   if (!ws || ws_bytes < L.total)
     return fail(NIQ_ERR_CAPACITY, "tree workspace too small: need " + std::to_string(L.total) + " bytes");
   if (!res->unknown_lower || !res->unknown_upper || res->unknown_capacity < 0)
@@ -140,6 +150,7 @@ niq_status tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg, const niq_t
     ClassifyArgs a{};
     a.lower = cur_lo; a.upper = cur_up; a.d = 3; a.n_dev = &C->n[level & 1]; a.offset = p->offset;
     a.codes = codes;
+    if (L.lw_bytes) { a.lw_buf = base + L.off_lw; a.lw_bytes = L.lw_bytes; }
     if (cfg && cfg->recheck_tol > 0.f) {
       a.recheck_tol = cfg->recheck_tol;
       a.recheck_cnt = &C->recheck_cnt;

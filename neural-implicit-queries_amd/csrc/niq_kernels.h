@@ -43,6 +43,8 @@ struct ClassifyArgs {
   int* recheck_idx;        // (recheck_cap,)
   int* recheck_cnt;        // device counter
   int recheck_cap;
+  void* lw_buf;            // precision mode, layer by layer: activation scratch
+  size_t lw_bytes;         //   (null: allocated per call, stream ordered)
 };
 
 // Fill the layer-0 input planes of one tile (features 0..15, zero padded).

@@ -7,15 +7,17 @@ The fused classify kernel comes in variants chosen by ``EvalCfg.variant``:
   tile (``TileCfgAlt``: twice the boxes per workgroup where LDS allows, fewer
   waves at width 512);
 * precision mode (BF16): 1 streams the packed bf16 weight copies, 2 loads the
-  FP32 weights once and splits them in registers.
+  FP32 weights once and splits them in registers, 3 runs layer by layer
+  (one batched launch per hidden layer over a chunk of boxes, activations
+  through HBM).
 
-Variants compute the same bounds; plain FP32 tiles change only the summation
-order inside the MFMA chain (within the 1e-5 relative bar of the plain form),
-and the precision-mode weight sources give the same splits (the e-plane bound
-differs by at most 2^-15 relative).  ``tune`` times each variant on boxes of
-the size the finest tree cells of ``split_depth`` have, keeps the fastest in
-``func.cfg.variant`` and caches the choice per (layer widths, activations,
-mode, precision).
+Variants compute the same bounds: the plain FP32 tiles sum every (box,
+feature) in the same order (bit-identical), the precision-mode weight sources
+give the same splits (the e-plane bound differs by at most 2^-15 relative)
+and variant 3 repeats variant 2's arithmetic exactly.  ``tune`` times each
+variant on boxes of the size the finest tree cells of ``split_depth`` have,
+keeps the fastest in ``func.cfg.variant`` and caches the choice per (layer
+widths, activations, mode, precision).
 """
 import numpy as np
 import torch
@@ -23,7 +25,12 @@ import torch
 from . import mlp as _mlp
 
 VARIANTS = (1, 2)
+VARIANTS_BF16 = (1, 2, 3)
 _cache = {}
+
+
+def variants_of(func):
+    return VARIANTS_BF16 if int(func.cfg.precision) == 1 else VARIANTS
 
 
 def _key(func, params):
@@ -47,7 +54,7 @@ def tune(func, params, split_depth=18, n_boxes=1 << 17, reps=3, seed=0):
     up = torch.as_tensor((c + h).astype(np.float32), device="cuda")
     times = {}
     saved = func.cfg.variant
-    for v in VARIANTS:
+    for v in variants_of(func):
         func.cfg.variant = v
         func.classify_box(params, lo, up)          # warm-up (and first-launch set-up)
         torch.cuda.synchronize()

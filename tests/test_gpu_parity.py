@@ -312,6 +312,20 @@ def test_bf16_inflation_at_finest_tree_cells(name):
     assert np.median(infl) < 1.6
 
 
+def test_bf16_layerwise_tree_equals_fused():
+    """The layer-by-layer precision mode inside a tree build (scratch in the
+    tree workspace, chunks of 32768 boxes) gives the fused kernel's tree."""
+    p = model("c2_relu_256x6")
+    f2, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision="bf16")
+    f3, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision="bf16")
+    f3.cfg.variant = 3
+    a, sa = kd_tree.build_tree(f2, p, LO, UP, split_depth=16, with_interior_nodes=True, with_exterior_nodes=True)
+    b, sb = kd_tree.build_tree(f3, p, LO, UP, split_depth=16, with_interior_nodes=True, with_exterior_nodes=True)
+    note(f"[bf16 layer-by-layer tree c2 depth 16] boxes {sa['n_classified']} / {sb['n_classified']}")
+    assert sa == sb
+    assert all(torch.equal(a[k], b[k]) for k in a)
+
+
 def test_bf16_tree_matches_fp32_up_to_inflation():
     p = model("c4_relu_512x8")
     f32, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
@@ -519,6 +533,12 @@ def test_bf16_weight_sources_agree(name):
     # |w1| + |w2| bounds |W| more tightly than |W| rounded up to bf16: the
     # in-register split is never looser, and at most a few percent tighter
     assert r.max() <= 1.0 + 1e-3 and r.min() > 0.95
+    # layer by layer (variant 3) repeats variant 2's products, order and
+    # rules, so its bounds are bit-identical
+    f.cfg.variant = 3
+    c3, b3 = f.classify_box(p, torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda"),
+                            return_bounds=True)
+    assert np.array_equal(c3.cpu().numpy(), out[2][0]) and np.array_equal(b3.cpu().numpy(), out[2][1])
     rng = np.random.default_rng(3)
     t = rng.uniform(0, 1, size=(lo.shape[0], 32, 3))
     pts = lo[:, None, :].astype(np.float64) + t * (up - lo)[:, None, :]
