@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 vs FP32 comparison")
     ap.add_argument("--precision-depth", type=int, default=21)
     ap.add_argument("--no-concurrent", action="store_true", help="skip the 64 concurrent trees")
+    ap.add_argument("--no-c3", action="store_true", help="skip the affine_all tree")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests in the serving trace (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
@@ -170,6 +171,26 @@ def serving_trace(n_req, rate, seed=0):
             "requests_per_s": n_req / wall, "wall_s": wall, "latency_ms": lat,
             "rays_merged_requests": merged, "trees_from_graphs": replays,
             "models": [m[0] for m in resident]}
+
+
+def c3_affine_all(depth=21):
+    """TanH MLP 3->128x8->1 (Xavier init, seed 1) under affine_all, whose
+    error terms grow to 3 + 1024 rows: the level-set tree to `depth`."""
+    import torch
+    from niq import models, implicit_mlp_utils, kd_tree
+    p = models.config_model("c3_tanh_128x8")
+    f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_all")
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    kd_tree.build_tree(f, p, lo, up, split_depth=9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, st = kd_tree.build_tree(f, p, lo, up, split_depth=depth)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lv = st["level_counts"]
+    return {"workload": f"levelset tree, TanH MLP 3->128x8->1, affine_all, split depth {depth}",
+            "seconds": dt, "boxes": st["n_classified"], "boxes_per_s": st["n_classified"] / dt,
+            "unknown_leaves": st["n_unknown"], "active_boxes_per_level": [min(lv), max(lv)]}
 
 
 def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):
@@ -315,6 +336,8 @@ def main():
     line["latency_ms"] = {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)}
     if not args.no_precision:
         line["precision"] = precision_compare(args.precision_depth)
+    if not args.no_c3:
+        line["c3_affine_all"] = c3_affine_all()
     if not args.no_concurrent:
         line["concurrent_trees"] = concurrent_trees()
     if args.serve_requests > 0:

@@ -165,8 +165,9 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
       L.c_base = (float)((4.1 * std::ldexp(1.0, -24) + g_chunk + g_sum) * 1.5);
       L.c_aff = (float)((3.1 * std::ldexp(1.0, -16) + g_chunk + g_sum) * 1.5);
       L.e_fix = (float)(1.0 + 1.5 * (gamma_n(33.0, std::ldexp(1.0, -23)) + g_sum));
-      // e |w1| + e |w2| in one 64-term chunk, and |W| <= (|w1| + |w2|) / (1 - 2^-16)
-      L.e_fix32 = (float)((1.0 + 1.5 * (gamma_n(65.0, std::ldexp(1.0, -23)) + g_sum)) *
+      // e |w1| + e |w2| summed in one accumulator over the whole K (2 in_pad
+      // non-negative terms), and |W| <= (|w1| + |w2|) / (1 - 2^-16)
+      L.e_fix32 = (float)((1.0 + 1.5 * gamma_n(2.0 * ip + 1.0, std::ldexp(1.0, -23))) *
                           (1.0 + std::ldexp(1.0, -15)));
     }
     m->host16.wmax = wmax16;

@@ -113,6 +113,29 @@ __device__ __forceinline__ void split_w8(const floatx4 lo, const floatx4 hi, bf1
   a2 = __builtin_bit_cast(bf16x8, u2 & 0x7fff7fffu);
 }
 
+// Base value split three ways for two features at once (packed pairs, low
+// half the first feature): x = h1 + h2 + h3 + r, |r| <= 2^-24 |x|.
+__device__ __forceinline__ void split3_pair(float a, float b, unsigned& h1, unsigned& h2, unsigned& h3) {
+  h1 = pk_bf16_rn(a, b);
+  const float ra = a - __uint_as_float(h1 << 16), rb = b - __uint_as_float(h1 & 0xffff0000u);
+  h2 = pk_bf16_rn(ra, rb);
+  h3 = pk_bf16_rn(ra - __uint_as_float(h2 << 16), rb - __uint_as_float(h2 & 0xffff0000u));
+}
+// Error-term value split two ways for two features.
+__device__ __forceinline__ void split2_pair(float a, float b, unsigned& h1, unsigned& h2) {
+  h1 = pk_bf16_rn(a, b);
+  h2 = pk_bf16_rn(a - __uint_as_float(h1 << 16), b - __uint_as_float(h1 & 0xffff0000u));
+}
+// Two non-negative floats rounded up to bf16, packed.
+__device__ __forceinline__ unsigned pk_bf16_up_nonneg(float a, float b) {
+  return ((__float_as_uint(a) + 0xffffu) >> 16) | ((__float_as_uint(b) + 0xffffu) & 0xffff0000u);
+}
+
+// Absolute floor added to every e-plane value: covers the split of values
+// below the FP32 normal range (at most 2^-126 each) whatever the conversion's
+// subnormal handling.
+constexpr float kTinyE = 1e-30f;
+
 // LDS layout: X[plane][box][k] as bf16 with the 16-byte chunk index
 // XOR-swizzled by box (conflict-free 16 B reads and 8 B writes).
 template <int NB, int WMAX>
@@ -150,7 +173,7 @@ __device__ __forceinline__ void store_split(unsigned short* Xs, int b, int k, co
     Xs[boff16<NB, WMAX>(4 + 2 * i, b, k)] = bf16_bits_rn(s[1 + i] - bf16_to_f(h));
     amag = amag + fabsf(s[1 + i]);
   }
-  const float e = up1(up1(err + c_base * fabsf(s[0]) + up1(c_aff * up1(amag * (1.f + 4.f * kU24)))));
+  const float e = up1(up1(err + c_base * fabsf(s[0]) + up1(c_aff * up1(amag * (1.f + 4.f * kU24))) + kTinyE));
   Xs[boff16<NB, WMAX>(3 + 2 * V, b, k)] = bf16_bits_up_nonneg(e);
 }
 
@@ -172,7 +195,19 @@ __device__ __forceinline__ void affine_rule_rigorous(int act, float* v) {
   rad = up1(up1((rad + 4.f * kU24 * fabsf(v[0])) * (1.f + 8.f * kU24)));
   const float lo = v[0] - rad, hi = v[0] + rad;
   float a, b, d;
-  act_rule(act, lo, hi, a, b, d);
+  if (act == ACT_RELU) {
+    // any slope in [0, 1] is sound here: the line's error is re-evaluated
+    // below, so the chord slope comes from the reciprocal estimate
+    if (lo >= 0.f) { a = 1.f; b = 0.f; d = 0.f; }
+    else if (hi <= 0.f) { a = 0.f; b = 0.f; d = 0.f; }
+    else {
+      a = fminf(fmaxf(hi * __builtin_amdgcn_rcpf(hi - lo), 0.f), 1.f);
+      b = -0.5f * (a * lo);
+      d = b;
+    }
+  } else {
+    act_rule(act, lo, hi, a, b, d);
+  }
   if (b == 0.f && d == 0.f && (a == 1.f || a == 0.f)) {
     // determinate feature (identity or zero): exact in FP32, no margin needed
 #pragma unroll
@@ -200,6 +235,35 @@ __device__ __forceinline__ void affine_rule_rigorous(int act, float* v) {
   // err' = a err + delta + margins for the rounding of base' and aff'
   const float err = a * v[V + 1] + d + m + 2.f * kU24 * (fabsf(pa) + fabsf(b) + amag);
   v[V + 1] = up1(up1(err * (1.f + 4.f * kU24)));
+}
+
+// The next layer's bf16 planes of 4 consecutive features of one box, as
+// packed pairs: base three ways, aff two ways, and the e plane
+// err + c_base |base| + c_aff sum |aff| (+ kTinyE) rounded up.
+template <int V> struct OutPlanes {
+  unsigned x[3][2];
+  unsigned a[V > 0 ? V : 1][2][2];
+  unsigned e[2];
+};
+
+template <int V>
+__device__ __forceinline__ void split_out4(float (&v)[4][V + 2], float cb_next, float ca_next, OutPlanes<V>& o) {
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    split3_pair(v[2 * q][0], v[2 * q + 1][0], o.x[0][q], o.x[1][q], o.x[2][q]);
+#pragma unroll
+    for (int i = 0; i < V; ++i) split2_pair(v[2 * q][1 + i], v[2 * q + 1][1 + i], o.a[i][0][q], o.a[i][1][q]);
+    float ev[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float* w = v[2 * q + h];
+      float amag = 0.f;
+#pragma unroll
+      for (int i = 0; i < V; ++i) amag = amag + fabsf(w[1 + i]);
+      ev[h] = up1(up1(w[V + 1] + cb_next * fabsf(w[0]) + up1(ca_next * up1(amag * (1.f + 4.f * kU24))) + kTinyE));
+    }
+    o.e[q] = pk_bf16_up_nonneg(ev[0], ev[1]);
+  }
 }
 
 // Fused forward in precision mode (affine forms only: KIND 0).  Xs holds
@@ -292,8 +356,9 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
             acc[f][1 + i][t] += c;
           }
           if constexpr (WS == WSRC_FP32) {
-            floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xe, z4, 0, 0, 0);
-            acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xe, c, 0, 0, 0);
+            // non-negative terms: one accumulator over the whole K
+            acc[f][NS][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xe, acc[f][NS][t], 0, 0, 0);
+            acc[f][NS][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xe, acc[f][NS][t], 0, 0, 0);
           } else {
             acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ly.Wa[o], xe, z4, 0, 0, 0);
           }
@@ -313,49 +378,33 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
 #pragma unroll
       for (int t = 0; t < BT; ++t) {
         const int b = t * 16 + ln;
-        unsigned short xs[3][4], as[V > 0 ? V : 1][2][4], es[4];
+        float v[4][V + 2];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          float v[V + 2];
-          v[0] = acc[f][0][t][j] + bias4[j];
+          v[j][0] = acc[f][0][t][j] + bias4[j];
 #pragma unroll
-          for (int i = 0; i < V; ++i) v[1 + i] = acc[f][1 + i][t][j];
-          v[V + 1] = up1(acc[f][NS][t][j] * (WS == WSRC_FP32 ? Ly.e_fix32 : Ly.e_fix));
-          if (last) {
-            if (ft == 0 && lg == 0 && j == 0) fin(v, b);
-            continue;
-          }
-          affine_rule_rigorous<V>(Ly.act, v);
-          const unsigned short h1 = bf16_bits_rn(v[0]);
-          const float r1 = v[0] - bf16_to_f(h1);
-          const unsigned short h2 = bf16_bits_rn(r1);
-          xs[0][j] = h1; xs[1][j] = h2; xs[2][j] = bf16_bits_rn(r1 - bf16_to_f(h2));
-          float amag = 0.f;
-#pragma unroll
-          for (int i = 0; i < V; ++i) {
-            const unsigned short h = bf16_bits_rn(v[1 + i]);
-            as[i][0][j] = h;
-            as[i][1][j] = bf16_bits_rn(v[1 + i] - bf16_to_f(h));
-            amag = amag + fabsf(v[1 + i]);
-          }
-          // next layer's contraction bound rides in the e plane
-          es[j] = bf16_bits_up_nonneg(up1(up1(v[V + 1] + cb_next * fabsf(v[0]) +
-                                              up1(ca_next * up1(amag * (1.f + 4.f * kU24))))));
+          for (int i = 0; i < V; ++i) v[j][1 + i] = acc[f][1 + i][t][j];
+          v[j][V + 1] = up1(acc[f][NS][t][j] * (WS == WSRC_FP32 ? Ly.e_fix32 : Ly.e_fix));
         }
-        if (!last) {
-          const int k = ft * 16 + lg * 4;
-#pragma unroll
-          for (int q = 0; q < 3; ++q)
-            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(q, b, k)]) = ushort4{xs[q][0], xs[q][1], xs[q][2], xs[q][3]};
-#pragma unroll
-          for (int i = 0; i < V; ++i) {
-            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k)]) =
-                ushort4{as[i][0][0], as[i][0][1], as[i][0][2], as[i][0][3]};
-            *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k)]) =
-                ushort4{as[i][1][0], as[i][1][1], as[i][1][2], as[i][1][3]};
-          }
-          *reinterpret_cast<ushort4*>(&Xs[boff16<NB, WMAX>(PB::E, b, k)]) = ushort4{es[0], es[1], es[2], es[3]};
+        if (last) {
+          if (ft == 0 && lg == 0) fin(v[0], b);
+          continue;
         }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) affine_rule_rigorous<V>(Ly.act, v[j]);
+        // next layer's planes; its contraction bound rides in the e plane
+        OutPlanes<V> o;
+        split_out4<V>(v, cb_next, ca_next, o);
+        const int k = ft * 16 + lg * 4;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          *reinterpret_cast<uint2*>(&Xs[boff16<NB, WMAX>(q, b, k)]) = uint2{o.x[q][0], o.x[q][1]};
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          *reinterpret_cast<uint2*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k)]) = uint2{o.a[i][0][0], o.a[i][0][1]};
+          *reinterpret_cast<uint2*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k)]) = uint2{o.a[i][1][0], o.a[i][1][1]};
+        }
+        *reinterpret_cast<uint2*>(&Xs[boff16<NB, WMAX>(PB::E, b, k)]) = uint2{o.e[0], o.e[1]};
       }
     }
   }

@@ -99,7 +99,7 @@ k_lw_prologue(const DevModelBf16* __restrict__ Mp, ClassifyArgs a, long long box
       B[(4 + 2 * p) * 512] = bf16_bits_rn(s[1 + p] - bf16_to_f(h));
       amag = amag + fabsf(s[1 + p]);
     }
-    const float ev = up1(up1(err + cb0 * fabsf(s[0]) + up1(ca0 * up1(amag * (1.f + 4.f * kU24)))));
+    const float ev = up1(up1(err + cb0 * fabsf(s[0]) + up1(ca0 * up1(amag * (1.f + 4.f * kU24))) + kTinyE));
     B[(3 + 2 * V) * 512] = bf16_bits_up_nonneg(ev);
   }
 }
@@ -216,10 +216,8 @@ k_lw_layer(const DevModelBf16* __restrict__ Mp, int l, ClassifyArgs a, long long
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, xp[3 + 2 * i], c, 0, 0, 0);
         acc[f][1 + i] += c;
       }
-      {
-        floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xp[NPL - 1], z4, 0, 0, 0);
-        acc[f][NS] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xp[NPL - 1], c, 0, 0, 0);
-      }
+      acc[f][NS] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xp[NPL - 1], acc[f][NS], 0, 0, 0);
+      acc[f][NS] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xp[NPL - 1], acc[f][NS], 0, 0, 0);
     }
     if (kb + 1 < nkb) store_stage(cur ^ 1);
     __syncthreads();
@@ -236,42 +234,28 @@ k_lw_layer(const DevModelBf16* __restrict__ Mp, int l, ClassifyArgs a, long long
     const int ftg = fgi * 8 + fg * 4 + f;       // global feature tile
     const int feat0 = ftg * 16 + lg * 4;
     const floatx4 bias4 = *reinterpret_cast<const floatx4*>(&Ly.bias[feat0]);
-    unsigned short xo[3][4], ao[V > 0 ? V : 1][2][4], eo[4];
+    float v[4][V + 2];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      float v[V + 2];
-      v[0] = acc[f][0][j] + bias4[j];
+      v[j][0] = acc[f][0][j] + bias4[j];
 #pragma unroll
-      for (int i = 0; i < V; ++i) v[1 + i] = acc[f][1 + i][j];
-      v[V + 1] = up1(acc[f][NS][j] * Ly.e_fix32);
-      affine_rule_rigorous<V>(Ly.act, v);
-      const unsigned short h1 = bf16_bits_rn(v[0]);
-      const float r1 = v[0] - bf16_to_f(h1);
-      const unsigned short h2 = bf16_bits_rn(r1);
-      xo[0][j] = h1; xo[1][j] = h2; xo[2][j] = bf16_bits_rn(r1 - bf16_to_f(h2));
-      float amag = 0.f;
-#pragma unroll
-      for (int i = 0; i < V; ++i) {
-        const unsigned short h = bf16_bits_rn(v[1 + i]);
-        ao[i][0][j] = h;
-        ao[i][1][j] = bf16_bits_rn(v[1 + i] - bf16_to_f(h));
-        amag = amag + fabsf(v[1 + i]);
-      }
-      eo[j] = bf16_bits_up_nonneg(up1(up1(v[V + 1] + cb_next * fabsf(v[0]) +
-                                          up1(ca_next * up1(amag * (1.f + 4.f * kU24))))));
+      for (int i = 0; i < V; ++i) v[j][1 + i] = acc[f][1 + i][j];
+      v[j][V + 1] = up1(acc[f][NS][j] * Ly.e_fix32);
+      affine_rule_rigorous<V>(Ly.act, v[j]);
     }
+    OutPlanes<V> o;
+    split_out4<V>(v, cb_next, ca_next, o);
     if (!live) continue;
     const int kbo = feat0 >> 5, ko = feat0 & 31;
     unsigned short* B = Xout + blk_off(btg, kbo, nkb_out, NPL) + ln * 32 + ko;
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-      *reinterpret_cast<ushort4*>(B + q * 512) = ushort4{xo[q][0], xo[q][1], xo[q][2], xo[q][3]};
+    for (int q = 0; q < 3; ++q) *reinterpret_cast<uint2*>(B + q * 512) = uint2{o.x[q][0], o.x[q][1]};
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      *reinterpret_cast<ushort4*>(B + (3 + 2 * i) * 512) = ushort4{ao[i][0][0], ao[i][0][1], ao[i][0][2], ao[i][0][3]};
-      *reinterpret_cast<ushort4*>(B + (4 + 2 * i) * 512) = ushort4{ao[i][1][0], ao[i][1][1], ao[i][1][2], ao[i][1][3]};
+      *reinterpret_cast<uint2*>(B + (3 + 2 * i) * 512) = uint2{o.a[i][0][0], o.a[i][0][1]};
+      *reinterpret_cast<uint2*>(B + (4 + 2 * i) * 512) = uint2{o.a[i][1][0], o.a[i][1][1]};
     }
-    *reinterpret_cast<ushort4*>(B + (NPL - 1) * 512) = ushort4{eo[0], eo[1], eo[2], eo[3]};
+    *reinterpret_cast<uint2*>(B + (NPL - 1) * 512) = uint2{o.e[0], o.e[1]};
   }
 }
 
@@ -320,10 +304,8 @@ k_lw_last(const DevModelBf16* __restrict__ Mp, int l, ClassifyArgs a, long long 
       c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, xp[3 + 2 * i], c, 0, 0, 0);
       acc[1 + i] += c;
     }
-    {
-      floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xp[NPL - 1], z4, 0, 0, 0);
-      acc[NS] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xp[NPL - 1], c, 0, 0, 0);
-    }
+    acc[NS] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xp[NPL - 1], acc[NS], 0, 0, 0);
+    acc[NS] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xp[NPL - 1], acc[NS], 0, 0, 0);
   }
   // output feature 0 sits in lanes 0..15 (lg == 0), slot j == 0
   if (lg != 0) return;
