@@ -24,6 +24,15 @@ import numpy as np
 
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes of one 262,144-box classify launch of the 3->256x6->1 model:
+# rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by what the counters read for a
+# plain 1 GiB copy (profiles/k_classify_c2_relu256x6_tree18.md)
+C2_CLASSIFY_TRAFFIC_BYTES = 19546602 + 1048576
+
+
+def progress(msg):
+    """A step of the bench finished (stderr; stdout carries the one JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -37,11 +46,16 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 vs FP32 comparison")
     ap.add_argument("--precision-depth", type=int, default=21)
+    ap.add_argument("--precision-ray-res", type=int, default=256,
+                    help="side of the 3->512x8->1 ray cast timed in FP32 and BF16 (0 skips it)")
     ap.add_argument("--no-concurrent", action="store_true", help="skip the 64 concurrent trees")
     ap.add_argument("--no-c3", action="store_true", help="skip the affine_all tree")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy a MIB-sized tensor (a plain copy kernel of known bytes, "
+                         "to scale the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests in the serving trace (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=5.0, help="Poisson arrival rate, requests/s")
     return ap.parse_args()
 
 
@@ -70,12 +84,12 @@ def cpu_baseline(params, target_s):
                       f"({boxes} boxes, {secs:.1f} s, NumPy FP32 with {threads} BLAS threads)"}
 
 
-def precision_compare(depth):
+def precision_compare(depth, ray_res=256):
     """ReLU MLP 3->512x8->1 (He init, seed 2): the level-set tree to `depth`
     with the plain FP32 contraction and with the BF16-MFMA contraction plus
     rigorous widening.  One warm-up and one timed build each."""
     import torch
-    from niq import models, implicit_mlp_utils, kd_tree
+    from niq import models, implicit_mlp_utils, kd_tree, queries, render
     p = models.config_model("c4_relu_512x8")
     lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
     res = {"workload": f"levelset tree, ReLU MLP 3->512x8->1, affine_fixed, split depth {depth}"}
@@ -92,6 +106,26 @@ def precision_compare(depth):
                      "unknown_leaves": st["n_unknown"],
                      "tflops": models.flops_per_box(dims, 5) * st["n_classified"] / dt / 1e12}
     res["speedup_bf16_over_fp32"] = res["fp32"]["seconds"] / res["bf16"]["seconds"]
+    if ray_res > 0:
+        # the same model's cast_rays (60 deg camera at distance 3, t in [0, 6])
+        eye = np.array([0., 0., 3.], np.float32)
+        look, upd, _ = render.look_at(eye)
+        roots, dirs = render.generate_camera_rays(eye, look, upd, res=ray_res, fov_deg=60.)
+        opts = queries.get_default_cast_opts()
+        opts["max_dist"] = 6.0
+        rays = {"res": ray_res}
+        for prec in ("fp32", "bf16"):
+            f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision=prec)
+            queries.cast_rays((f,), (p,), roots[:1024], dirs[:1024], opts)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, hid, cnt, _ = queries.cast_rays((f,), (p,), roots, dirs, opts)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            rays[prec] = {"rays_per_s": ray_res ** 2 / dt, "seconds": dt, "hits": int((hid > 0).sum().item()),
+                          "ray_steps": int(cnt.sum().item())}
+        rays["speedup_bf16_over_fp32"] = rays["fp32"]["seconds"] / rays["bf16"]["seconds"]
+        res["rays"] = rays
     return res
 
 
@@ -103,10 +137,12 @@ SERVE_SHAPES = (("c1 3->32x3->1 relu", [32] * 3, "relu", "he"),
 def serving_trace(n_req, rate, seed=0):
     """A reduced serving trace: eight resident MLPs (seeds 10-17, shapes drawn
     from the C1-C3 shapes) under affine_fixed; Poisson arrivals (seed 0) of
-    50 % ray-cast tiles (side U{64..512} of the 60-degree camera at distance 3),
-    30 % closest-point batches (U{1..512} points in [-1,1]^3, eps 0.01) and
-    20 % level-set trees (split depth U{12..21}); memory cap 200 GB.  Reports
-    p50/p99 latency per kind and overall, and requests/s."""
+    50 % ray-cast tiles (side U{64..256} of the 60-degree camera at distance 3),
+    30 % closest-point batches (U{1..64} points in [-1,1]^3, eps 0.05) and
+    20 % level-set trees (split depth U{12..21}); memory cap 200 GB.  The tile
+    sides and closest-point batches are scaled down from 64-512 px and
+    1-65,536 points so that the trace runs in about a minute.  Reports p50/p99
+    latency per kind and overall, and requests/s."""
     import torch
     from niq import models, implicit_mlp_utils, queries, render, serve
     rng = np.random.default_rng(seed)
@@ -128,10 +164,10 @@ def serving_trace(n_req, rate, seed=0):
         u = rng.uniform()
         k = int(rng.integers(0, len(resident)))
         if u < 0.5:
-            side = int(rng.integers(64, 513))
+            side = int(rng.integers(64, 257))
             plan.append(("rays", k, side))
         elif u < 0.8:
-            plan.append(("closest_point", k, int(rng.integers(1, 513))))
+            plan.append(("closest_point", k, int(rng.integers(1, 65))))
         else:
             plan.append(("tree", k, int(rng.integers(12, 22))))
     # inputs built before the clock starts (resident in HBM)
@@ -157,19 +193,21 @@ def serving_trace(n_req, rate, seed=0):
             if kind == "rays":
                 reqs.append(srv.submit_rays((f,), (p,), inp[0], inp[1], opts))
             elif kind == "closest_point":
-                reqs.append(srv.submit_closest_point(f, p, lo, up, inp, eps=0.01))
+                reqs.append(srv.submit_closest_point(f, p, lo, up, inp, eps=0.05))
             else:
                 reqs.append(srv.submit_tree(f, p, lo, up, split_depth=arg))
-        for r in reqs:
+        for i, r in enumerate(reqs):
             r.result()
+            if (i + 1) % 20 == 0:
+                progress(f"serving: {i + 1}/{len(reqs)} requests done")
         wall = time.perf_counter() - t0
     lat = serve.latency_summary(reqs)
     merged = sum(1 for r in reqs if r.kind == "rays" and r.info["n_batched"] > 1)
     replays = sum(1 for r in reqs if r.kind == "tree" and r.info["graph_replay"])
     return {"workload": f"{n_req} requests, Poisson {rate:g}/s (seed {seed}), 8 resident MLPs, "
-                        "50% ray tiles 64-512 px, 30% closest-point 1-512 pts, 20% trees depth 12-21",
+                        "50% ray tiles 64-256 px, 30% closest-point 1-64 pts (eps 0.05), 20% trees depth 12-21",
             "requests_per_s": n_req / wall, "wall_s": wall, "latency_ms": lat,
-            "rays_merged_requests": merged, "trees_from_graphs": replays,
+            "rays_merged_requests": merged, "arrival_rate_per_s": rate, "trees_from_graphs": replays,
             "models": [m[0] for m in resident]}
 
 
@@ -252,6 +290,15 @@ def main():
     import torch
     from niq import models, implicit_mlp_utils, kd_tree, queries, render
 
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        progress(f"copy calibration: 3 copies of {args.copy_calibration} MiB")
+
     name = "c2_relu_256x6"
     params = models.config_model(name)
     func, params = implicit_mlp_utils.implicit_from_params(params, "affine_fixed")
@@ -309,6 +356,13 @@ def main():
     torch.cuda.synchronize()
     ray_s = time.perf_counter() - t1
     ray_steps = int(cnt.sum().item())
+    func16, _ = implicit_mlp_utils.implicit_from_params(params, "affine_fixed", precision="bf16")
+    queries.cast_rays((func16,), (params,), roots[:4096], dirs[:4096], opts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, hid16, cnt16, _ = queries.cast_rays((func16,), (params,), roots, dirs, opts)
+    torch.cuda.synchronize()
+    ray16_s = time.perf_counter() - t1
 
     line = {
         "metric": "boxes bounded/s (level-set tree, affine_fixed FP32)",
@@ -327,21 +381,30 @@ def main():
                    "boxes_per_step": n_boxes // args.steps, "parallelism": "single GPU"},
         "rays": {"rays_per_s": args.ray_res ** 2 / ray_s, "ray_steps_per_s": ray_steps / ray_s,
                  "res": args.ray_res, "seconds": ray_s, "hits": int((hid > 0).sum().item()),
-                 "n_evals": n_evals},
+                 "n_evals": n_evals,
+                 "bf16": {"rays_per_s": args.ray_res ** 2 / ray16_s, "seconds": ray16_s,
+                          "hits": int((hid16 > 0).sum().item()), "ray_steps": int(cnt16.sum().item()),
+                          "speedup_over_fp32": ray_s / ray16_s}},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+                     "traffic": C2_CLASSIFY_TRAFFIC_BYTES if n_cls == 1 << 18 else None,
                      "kernel": "k_classify (affine_fixed, W=256)", "launch_ms": cls_ms,
                      "flops_per_box": flops_box, "boxes_per_launch": n_cls},
     }
     line["latency_ms"] = {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)}
+    progress(f"tree: {boxes_per_s:.4g} boxes/s; rays: {args.ray_res ** 2 / ray_s:.4g} rays/s")
     if not args.no_precision:
-        line["precision"] = precision_compare(args.precision_depth)
+        line["precision"] = precision_compare(args.precision_depth, args.precision_ray_res)
+        progress(f"precision: bf16 {line['precision']['speedup_bf16_over_fp32']:.3f}x over fp32")
     if not args.no_c3:
         line["c3_affine_all"] = c3_affine_all()
+        progress(f"c3 affine_all: {line['c3_affine_all']['boxes_per_s']:.4g} boxes/s")
     if not args.no_concurrent:
         line["concurrent_trees"] = concurrent_trees()
+        progress(f"concurrent trees: {line['concurrent_trees']['boxes_per_s_served']:.4g} boxes/s served")
     if args.serve_requests > 0:
         line["serving"] = serving_trace(args.serve_requests, args.serve_rate)
+        progress(f"serving: {line['serving']['requests_per_s']:.4g} requests/s")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(params, args.cpu_seconds)
     print(json.dumps(line), flush=True)

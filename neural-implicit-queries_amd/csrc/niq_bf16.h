@@ -246,8 +246,8 @@ template <int V> struct OutPlanes {
   unsigned e[2];
 };
 
-template <int V>
-__device__ __forceinline__ void split_out4(float (&v)[4][V + 2], float cb_next, float ca_next, OutPlanes<V>& o) {
+template <int V, int NV>
+__device__ __forceinline__ void split_out4(float (&v)[4][NV], float cb_next, float ca_next, OutPlanes<V>& o) {
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     split3_pair(v[2 * q][0], v[2 * q + 1][0], o.x[0][q], o.x[1][q], o.x[2][q]);
@@ -269,11 +269,15 @@ __device__ __forceinline__ void split_out4(float (&v)[4][V + 2], float cb_next, 
 // Fused forward in precision mode (affine forms only: KIND 0).  Xs holds
 // PlanesBf16<V>::NPL bf16 planes.  fin(v, b) gets base, aff_i, err (FP32,
 // err rigorous) of output feature 0.
-template <int V, int NB, int NW, int WMAX, int WS, class Fin>
+// NP plain point rows (for the ray caster's hit test) ride after the range
+// form, each split NPS ways (planes PB::NPL + r * NPS + s); they get the
+// plain activation and reach `fin` in v[V + 2 + r].
+template <int V, int NB, int NW, int WMAX, int WS, int NP = 0, int NPS = 3, class Fin>
 __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned short* __restrict__ Xs, Fin&& fin) {
   using PB = PlanesBf16<V>;
   constexpr int NS = V + 1;
   constexpr int R = NS + 1;   // accumulators: base, aff_i, e
+  constexpr int RA = R + NP;  // ... and the point rows
   constexpr int BT = NB / 16;
   constexpr int FTMAX = (WMAX / 16 + NW - 1) / NW;
   const int lane = threadIdx.x & 63;
@@ -287,11 +291,11 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
     const int nkb = Ly.in_pad >> 5;
     const int nft = Ly.out_pad >> 4;
     const bool last = (l == M.n_layers - 1);
-    floatx4 acc[FTMAX][R][BT];
+    floatx4 acc[FTMAX][RA][BT];
 #pragma unroll
     for (int f = 0; f < FTMAX; ++f)
 #pragma unroll
-      for (int p = 0; p < R; ++p)
+      for (int p = 0; p < RA; ++p)
 #pragma unroll
         for (int t = 0; t < BT; ++t) acc[f][p][t] = z4;
 
@@ -362,6 +366,27 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
           } else {
             acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ly.Wa[o], xe, z4, 0, 0, 0);
           }
+          // plain point rows (hit test): the same split products, no bound
+#pragma unroll
+          for (int r = 0; r < NP; ++r) {
+            const int pl = PB::NPL + r * NPS;
+            const bf16x8 q1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl, b, k0)]);
+            const bf16x8 q2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl + 1, b, k0)]);
+            floatx4 c;
+            if constexpr (NPS == 3) {
+              const bf16x8 q3 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl + 2, b, k0)]);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q3, z4, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q2, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, q1, c, 0, 0, 0);
+            } else {
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, q1, z4, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q2, c, 0, 0, 0);
+            }
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q2, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q1, c, 0, 0, 0);
+            acc[f][R + r][t] += c;
+          }
         }
       }
     }
@@ -378,24 +403,43 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
 #pragma unroll
       for (int t = 0; t < BT; ++t) {
         const int b = t * 16 + ln;
-        float v[4][V + 2];
+        float v[4][V + 2 + NP];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           v[j][0] = acc[f][0][t][j] + bias4[j];
 #pragma unroll
           for (int i = 0; i < V; ++i) v[j][1 + i] = acc[f][1 + i][t][j];
           v[j][V + 1] = up1(acc[f][NS][t][j] * (WS == WSRC_FP32 ? Ly.e_fix32 : Ly.e_fix));
+#pragma unroll
+          for (int r = 0; r < NP; ++r) v[j][V + 2 + r] = acc[f][R + r][t][j] + bias4[j];
         }
         if (last) {
           if (ft == 0 && lg == 0) fin(v[0], b);
           continue;
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) affine_rule_rigorous<V>(Ly.act, v[j]);
+        for (int j = 0; j < 4; ++j) {
+          affine_rule_rigorous<V>(Ly.act, v[j]);
+#pragma unroll
+          for (int r = 0; r < NP; ++r) v[j][V + 2 + r] = act_point(Ly.act, v[j][V + 2 + r]);
+        }
         // next layer's planes; its contraction bound rides in the e plane
         OutPlanes<V> o;
         split_out4<V>(v, cb_next, ca_next, o);
         const int k = ft * 16 + lg * 4;
+#pragma unroll
+        for (int r = 0; r < NP; ++r) {
+          const int pl = PB::NPL + r * NPS;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            unsigned h1, h2, h3 = 0;
+            if constexpr (NPS == 3) split3_pair(v[2 * q][V + 2 + r], v[2 * q + 1][V + 2 + r], h1, h2, h3);
+            else split2_pair(v[2 * q][V + 2 + r], v[2 * q + 1][V + 2 + r], h1, h2);
+            *reinterpret_cast<unsigned*>(&Xs[boff16<NB, WMAX>(pl, b, k + 2 * q)]) = h1;
+            *reinterpret_cast<unsigned*>(&Xs[boff16<NB, WMAX>(pl + 1, b, k + 2 * q)]) = h2;
+            if constexpr (NPS == 3) *reinterpret_cast<unsigned*>(&Xs[boff16<NB, WMAX>(pl + 2, b, k + 2 * q)]) = h3;
+          }
+        }
 #pragma unroll
         for (int q = 0; q < 3; ++q)
           *reinterpret_cast<uint2*>(&Xs[boff16<NB, WMAX>(q, b, k)]) = uint2{o.x[q][0], o.x[q][1]};
