@@ -46,8 +46,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 vs FP32 comparison")
     ap.add_argument("--precision-depth", type=int, default=21)
-    ap.add_argument("--precision-ray-res", type=int, default=256,
-                    help="side of the 3->512x8->1 ray cast timed in FP32 and BF16 (0 skips it)")
+    ap.add_argument("--precision-ray-res", type=int, default=512,
+                    help="side of the ray cast timed in FP32 and BF16 (0 skips it)")
     ap.add_argument("--no-concurrent", action="store_true", help="skip the 64 concurrent trees")
     ap.add_argument("--no-c3", action="store_true", help="skip the affine_all tree")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -84,12 +84,13 @@ def cpu_baseline(params, target_s):
                       f"({boxes} boxes, {secs:.1f} s, NumPy FP32 with {threads} BLAS threads)"}
 
 
-def precision_compare(depth, ray_res=256):
+def precision_compare(depth, ray_res=512):
     """ReLU MLP 3->512x8->1 (He init, seed 2): the level-set tree to `depth`
     with the plain FP32 contraction and with the BF16-MFMA contraction plus
     rigorous widening.  One warm-up and one timed build each."""
     import torch
     from niq import models, implicit_mlp_utils, kd_tree, queries, render
+    from niq import mlp as nmlp
     p = models.config_model("c4_relu_512x8")
     lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
     res = {"workload": f"levelset tree, ReLU MLP 3->512x8->1, affine_fixed, split depth {depth}"}
@@ -107,13 +108,17 @@ def precision_compare(depth, ray_res=256):
                      "tflops": models.flops_per_box(dims, 5) * st["n_classified"] / dt / 1e12}
     res["speedup_bf16_over_fp32"] = res["fp32"]["seconds"] / res["bf16"]["seconds"]
     if ray_res > 0:
-        # the same model's cast_rays (60 deg camera at distance 3, t in [0, 6])
+        # cast_rays in both forms.  On this He-init net the precision mode's
+        # a-priori widening exceeds the function's scale, so it certifies no
+        # step; rays are timed on the reference's trained fox SDF instead
+        # (60 deg camera at distance 3, t in [0, 6]).
+        p = nmlp.load(os.path.join(ROOT, "tests", "golden", "fox.npz"))
         eye = np.array([0., 0., 3.], np.float32)
         look, upd, _ = render.look_at(eye)
         roots, dirs = render.generate_camera_rays(eye, look, upd, res=ray_res, fov_deg=60.)
         opts = queries.get_default_cast_opts()
         opts["max_dist"] = 6.0
-        rays = {"res": ray_res}
+        rays = {"model": "fox (the reference's sample SDF)", "res": ray_res}
         for prec in ("fp32", "bf16"):
             f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision=prec)
             queries.cast_rays((f,), (p,), roots[:1024], dirs[:1024], opts)
@@ -356,13 +361,7 @@ def main():
     torch.cuda.synchronize()
     ray_s = time.perf_counter() - t1
     ray_steps = int(cnt.sum().item())
-    func16, _ = implicit_mlp_utils.implicit_from_params(params, "affine_fixed", precision="bf16")
-    queries.cast_rays((func16,), (params,), roots[:4096], dirs[:4096], opts)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    _, hid16, cnt16, _ = queries.cast_rays((func16,), (params,), roots, dirs, opts)
-    torch.cuda.synchronize()
-    ray16_s = time.perf_counter() - t1
+
 
     line = {
         "metric": "boxes bounded/s (level-set tree, affine_fixed FP32)",
@@ -381,10 +380,7 @@ def main():
                    "boxes_per_step": n_boxes // args.steps, "parallelism": "single GPU"},
         "rays": {"rays_per_s": args.ray_res ** 2 / ray_s, "ray_steps_per_s": ray_steps / ray_s,
                  "res": args.ray_res, "seconds": ray_s, "hits": int((hid > 0).sum().item()),
-                 "n_evals": n_evals,
-                 "bf16": {"rays_per_s": args.ray_res ** 2 / ray16_s, "seconds": ray16_s,
-                          "hits": int((hid16 > 0).sum().item()), "ray_steps": int(cnt16.sum().item()),
-                          "speedup_over_fp32": ray_s / ray16_s}},
+                 "n_evals": n_evals},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
                      "traffic": C2_CLASSIFY_TRAFFIC_BYTES if n_cls == 1 << 18 else None,

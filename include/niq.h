@@ -176,6 +176,43 @@ niq_status niq_cast_rays(const niq_model* const* models, int n_models, const niq
                          int32_t* out_count, void* workspace, size_t workspace_bytes,
                          int64_t* n_evals, void* stream);
 
+/* ---- frustum ray casting -----------------------------------------------------
+ * Replaces cast_rays_frustum (src/queries.py:454-587) with its iteration
+ * (:178-364), subdivision (:371-432) and output (:434-452).  The camera is the
+ * reference's cam_params (root_pos, look_dir, up_dir, left_dir, fov_x, fov_y,
+ * res_x, res_y); outputs out_t, out_hit_id, out_count are (res_x, res_y)
+ * row-major, indexed [x][y] as the reference's; *n_evals receives its N_evals
+ * (frustum-array sizes summed over iterations).  Synchronising. */
+typedef struct niq_frustum_cam {
+  float root[3], look[3], up[3], left[3];
+  float fov_x, fov_y;       /* degrees */
+  int res_x, res_y;
+} niq_frustum_cam;
+
+typedef struct niq_frustum_opts {
+  niq_cast_opts cast;       /* hit_eps .. init_step as for niq_cast_rays */
+  float refine_width_fac;
+  int n_side_init;
+} niq_frustum_opts;
+
+niq_status niq_cast_rays_frustum(const niq_model* const* models, int n_models, const niq_eval_cfg* cfg,
+                                 const niq_frustum_cam* cam, const niq_frustum_opts* opts,
+                                 float* out_t, int32_t* out_hit_id, int32_t* out_count,
+                                 int64_t* n_evals, void* stream);
+
+/* ---- intersection of two functions --------------------------------------------
+ * Replaces find_any_intersection (src/kd_tree.py:402-655): a breadth-first
+ * kd-tree search over [lower, upper] for a point inside both functions (both
+ * values < 0).  *found = 1 and loc = that point, from the first node in the
+ * reference's level order that certifies one; else *found = 0 and loc =
+ * (-777, -777, -777).  *n_processed counts the nodes searched.  Synchronising
+ * (once per level). */
+niq_status niq_find_any_intersection(const niq_model* model_a, const niq_eval_cfg* cfg_a,
+                                     const niq_model* model_b, const niq_eval_cfg* cfg_b,
+                                     const float lower[3], const float upper[3], float eps,
+                                     int32_t* found, float loc[3], int64_t* n_processed,
+                                     void* stream);
+
 /* ---- closest point ---------------------------------------------------------
  * Replaces closest_point / closest_point_iter (src/kd_tree.py:766-802, 659-763)
  * with the same batch-stack order (B = batch_process_size items per round), so

@@ -252,9 +252,20 @@ niq_status launch_points_dev_w(const niq_model* m, const float* x, const int* n_
   const long long tiles = (max_n + NB - 1) / NB;
   if (tiles == 0) return NIQ_OK;
   const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
-  hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, x, n_dev, mult, out);
+  hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, x, n_dev, mult, max_n, out);
   HIPCHK(hipGetLastError());
   return NIQ_OK;
 }
 }  // namespace
+
+niq_status niqi::points_dev(const niq_model* m, const float* x, const int* n_dev, int mult, long long max_n,
+                            float* out, hipStream_t s) {
+  switch (m->wt) {
+    case 64: return launch_points_dev_w<64>(m, x, n_dev, mult, max_n, out, s);
+    case 128: return launch_points_dev_w<128>(m, x, n_dev, mult, max_n, out, s);
+    case 256: return launch_points_dev_w<256>(m, x, n_dev, mult, max_n, out, s);
+    case 512: return launch_points_dev_w<512>(m, x, n_dev, mult, max_n, out, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
 

@@ -141,6 +141,10 @@ niq_status classify_rows(const niq_model* m, const niq_eval_cfg* cfg, const Clas
 // ... with the optional FP64 re-test list in stream-ordered scratch.
 niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, int nvec, ClassifyArgs a,
                                  long long n, hipStream_t s);
+// Plain evaluation of mult * (*n_dev) points (n_dev null: max_n points).
+// Defined in api_cp_mc.hip.
+niq_status points_dev(const niq_model* m, const float* x, const int* n_dev, int mult, long long max_n, float* out,
+                      hipStream_t s);
 // niq_tree_enqueue with the root box optionally read from the device (6
 // floats, lower then upper), so that one captured build serves any root box.
 // Defined in api_tree.hip.

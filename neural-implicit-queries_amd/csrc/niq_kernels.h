@@ -203,15 +203,16 @@ k_points(const DevModel* __restrict__ Mp, const float* __restrict__ x, int d, lo
   }
 }
 
-// Point values with a device-resident count: n = (*n_dev) * mult points (3D).
+// Point values of n = (*n_dev) * mult points (3D), or of n_host points when
+// n_dev is null.
 template <int NB, int NW, int WMAX>
 __global__ void __launch_bounds__(NW * 64)
 k_points_dev(const DevModel* __restrict__ Mp, const float* __restrict__ x, const int* __restrict__ n_dev, int mult,
-             float* __restrict__ out) {
+             long long n_host, float* __restrict__ out) {
   using PL = Planes<2, 0, 1>;
   constexpr int NT = NW * 64;
   __shared__ float X[NB * WMAX];
-  const long long n = (long long)(*n_dev) * mult;
+  const long long n = n_dev ? (long long)(*n_dev) * mult : n_host;
   const long long ntiles = (n + NB - 1) / NB;
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const long long i0 = tile * NB;

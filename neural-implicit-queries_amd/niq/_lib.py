@@ -27,7 +27,7 @@ EXPORTS = ("niq_model_create", "niq_model_destroy", "niq_model_device_bytes",
            "niq_classify_boxes", "niq_classify_general_boxes", "niq_eval_points",
            "niq_tree_workspace_bytes", "niq_tree_build", "niq_tree_enqueue",
            "niq_tree_counters_bytes", "niq_tree_decode", "niq_cast_workspace_bytes",
-           "niq_cast_rays", "niq_closest_point", "niq_mc_extract", "niq_mc_table",
+           "niq_cast_rays", "niq_cast_rays_frustum", "niq_find_any_intersection", "niq_closest_point", "niq_mc_extract", "niq_mc_table",
            "niq_serve_create", "niq_serve_destroy", "niq_serve_submit_rays", "niq_serve_submit_tree",
            "niq_serve_submit_closest_point", "niq_serve_poll", "niq_serve_bytes_in_use",
            "niq_last_error", "niq_version")
@@ -78,6 +78,16 @@ class CastOpts(ctypes.Structure):
                 ("interval_shrink_fac", ctypes.c_float), ("init_step", ctypes.c_float)]
 
 
+class FrustumCam(ctypes.Structure):
+    _fields_ = [("root", ctypes.c_float * 3), ("look", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("left", ctypes.c_float * 3), ("fov_x", ctypes.c_float), ("fov_y", ctypes.c_float),
+                ("res_x", ctypes.c_int), ("res_y", ctypes.c_int)]
+
+
+class FrustumOpts(ctypes.Structure):
+    _fields_ = [("cast", CastOpts), ("refine_width_fac", ctypes.c_float), ("n_side_init", ctypes.c_int)]
+
+
 class ServeInfo(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int),
                 ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
@@ -112,6 +122,9 @@ def load_library():
         "niq_cast_workspace_bytes": (S, [P]),
         "niq_cast_rays": (I, [P, I, P, P, P, I64, P, P, P, P, P, S, ctypes.POINTER(I64), P]),
         "niq_closest_point": (I, [P, P, P, P, P, I64, F, I, P, P, ctypes.POINTER(I64), P]),
+        "niq_cast_rays_frustum": (I, [P, I, P, P, P, P, P, P, ctypes.POINTER(I64), P]),
+        "niq_find_any_intersection": (I, [P, P, P, P, P, P, F, ctypes.POINTER(ctypes.c_int32), P,
+                                          ctypes.POINTER(I64), P]),
         "niq_mc_extract": (I, [P, P, P, I64, I, P, I64, ctypes.POINTER(I64), P]),
         "niq_mc_table": (I, [P]),
         "niq_serve_create": (I, [I, S, I64, ctypes.POINTER(P)]),

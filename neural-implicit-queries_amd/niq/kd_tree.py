@@ -179,6 +179,32 @@ def closest_point(func, params, lower, upper, query_points, eps=0.001, batch_pro
     return dist, loc
 
 
+def find_any_intersection(func_tuple, params_tuple, lower, upper, eps, viz_nodes=False):
+    """src/kd_tree.py:558-655: (found, 1, 2, loc (3,)) for a pair of
+    functions; loc is (-777, -777, -777) when none is found.  The node
+    visualisation output (viz_nodes) is not provided."""
+    if len(func_tuple) != 2 or len(params_tuple) != 2:
+        raise ValueError("intersection supports pairwise only as written")
+    if viz_nodes:
+        raise NotImplementedError("viz_nodes output is not provided")
+    _lib.require_gpu()
+    lib = _lib.load_library()
+    mA, mB = _mlp.device_model(params_tuple[0]), _mlp.device_model(params_tuple[1])
+    lo = (ctypes.c_float * 3)(*[float(v) for v in torch.as_tensor(lower).reshape(-1).tolist()])
+    up = (ctypes.c_float * 3)(*[float(v) for v in torch.as_tensor(upper).reshape(-1).tolist()])
+    found = ctypes.c_int32(0)
+    loc = (ctypes.c_float * 3)()
+    n_proc = ctypes.c_int64(0)
+    _lib.check(lib.niq_find_any_intersection(mA.handle, ctypes.byref(func_tuple[0].cfg), mB.handle,
+                                             ctypes.byref(func_tuple[1].cfg), lo, up, float(eps),
+                                             ctypes.byref(found), loc, ctypes.byref(n_proc),
+                                             _lib.stream_handle()))
+    find_any_intersection.last_n_processed = int(n_proc.value)
+    if found.value:
+        return True, 1, 2, torch.tensor(list(loc), dtype=torch.float32)
+    return False, 0, 0, torch.tensor([-777., -777., -777.], dtype=torch.float32)
+
+
 def extract_cells(params, node_lower, node_upper, n_leaves, n_subcell_depth):
     """Marching cubes over the first n_leaves boxes: (T, 3, 3) triangles."""
     lib = _lib.load_library()

@@ -80,3 +80,43 @@ def cast_rays(funcs_tuple, params_tuple, roots, dirs, opts, stream=None):
                                  _lib.ptr(ws), ws_bytes, ctypes.byref(n_evals),
                                  _lib.stream_handle(stream)))
     return out_t, out_hit, out_count, int(n_evals.value)
+
+
+def cast_rays_frustum(funcs_tuple, params_tuple, cam_params, opts, stream=None):
+    """src/queries.py:454-587: (t, hit_id, count) of shape (res_x, res_y),
+    indexed [x, y], and N_evals.  ``cam_params`` is the reference's
+    (root_pos, look_dir, up_dir, left_dir, fov_x, fov_y, res_x, res_y)."""
+    if not isinstance(funcs_tuple, (tuple, list)):
+        funcs_tuple = (funcs_tuple,)
+    if not isinstance(params_tuple, (tuple, list)):
+        params_tuple = (params_tuple,)
+    if len(funcs_tuple) != len(params_tuple):
+        raise ValueError("funcs_tuple and params_tuple lengths differ")
+    if len({f.cfg.mode for f in funcs_tuple}) != 1:
+        raise ValueError("all functions of one cast must use the same range-analysis mode")
+    _lib.require_gpu()
+    lib = _lib.load_library()
+    root_pos, look_dir, up_dir, left_dir, fov_x, fov_y, res_x, res_y = cam_params
+    cam = _lib.FrustumCam()
+    for k in range(3):
+        cam.root[k] = float(np.asarray(root_pos, np.float32).reshape(-1)[k])
+        cam.look[k] = float(np.asarray(look_dir, np.float32).reshape(-1)[k])
+        cam.up[k] = float(np.asarray(up_dir, np.float32).reshape(-1)[k])
+        cam.left[k] = float(np.asarray(left_dir, np.float32).reshape(-1)[k])
+    cam.fov_x, cam.fov_y = float(np.float32(fov_x)), float(np.float32(fov_y))
+    cam.res_x, cam.res_y = int(res_x), int(res_y)
+    fo = _lib.FrustumOpts()
+    fo.cast = cast_opts_struct(opts)
+    fo.refine_width_fac = float(opts['refine_width_fac'])
+    fo.n_side_init = int(opts['n_side_init'])
+    models = [_mlp.device_model(p) for p in params_tuple]
+    handles = (ctypes.c_void_p * len(models))(*[m.handle.value for m in models])
+    dev = torch.device("cuda")
+    out_t = torch.empty((cam.res_x, cam.res_y), dtype=torch.float32, device=dev)
+    out_hit = torch.empty((cam.res_x, cam.res_y), dtype=torch.int32, device=dev)
+    out_count = torch.empty((cam.res_x, cam.res_y), dtype=torch.int32, device=dev)
+    n_evals = ctypes.c_int64(0)
+    _lib.check(lib.niq_cast_rays_frustum(handles, len(models), ctypes.byref(funcs_tuple[0].cfg), ctypes.byref(cam),
+                                         ctypes.byref(fo), _lib.ptr(out_t), _lib.ptr(out_hit), _lib.ptr(out_count),
+                                         ctypes.byref(n_evals), _lib.stream_handle(stream)))
+    return out_t, out_hit, out_count, int(n_evals.value)

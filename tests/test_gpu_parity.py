@@ -25,6 +25,7 @@ import pytest
 import torch
 
 from oracle import funcs as ofuncs, tree as otree, rays as orays, mlp as omlp, affine as oaff
+from oracle import frustum as ofr, intersect as oisect
 from niq import models, mlp as nmlp, implicit_mlp_utils, kd_tree, queries, render
 
 pytestmark = pytest.mark.gpu
@@ -256,6 +257,106 @@ def test_rays_two_functions():
     assert same.mean() >= 0.99
 
 
+# ------------------------------------------------------------ frustum rays
+def frust_cam(res, eye=(0., 0., 3.)):
+    eye = np.array(eye, np.float32)
+    look, up, left = render.look_at(eye)
+    return (eye, look, up, left, 60.0, 60.0, res, res)
+
+
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "small_tanh", "fox"])
+def test_frustum_matches_oracle(name):
+    """Frustum casting (src/queries.py:454-587): per pixel (t, hit_id, count)
+    identical to the oracle's for >= 99 % of pixels (near-threshold FP32
+    decisions aside), and N_evals identical when every pixel is."""
+    note(f"[frustum] {name}")
+    p = model(name)
+    f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    opts["n_side_init"] = 8
+    cam = frust_cam(64)
+    t, h, c, ne = queries.cast_rays_frustum((f,), (p,), cam, opts)
+    rt, rh, rc, rne = ofr.cast_rays_frustum((ofuncs.OracleFunc(p),), *cam, opts)
+    t, h, c = t.cpu().numpy(), h.cpu().numpy(), c.cpu().numpy()
+    same = (t == rt) & (h == rh) & (c == rc)
+    note(f"  {same.mean() * 100:.2f}% pixels identical; hits {int((h > 0).sum())} / {int((rh > 0).sum())}; "
+         f"N_evals {ne} / {rne}")
+    assert same.mean() >= 0.99
+    if same.all():
+        assert ne == rne
+
+
+def test_frustum_plane_closed_form():
+    note("[frustum plane]")
+    nrm = np.array([0.3, -0.5, 0.81], np.float64)
+    nrm /= np.linalg.norm(nrm)
+    A1 = np.concatenate([np.eye(3), -np.eye(3)], axis=1)
+    A2 = np.concatenate([0.5 * nrm, -0.5 * nrm])[:, None]
+    p = nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-0.25]),
+                         nmlp.squeeze_last()])
+    f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    opts["n_side_init"] = 8
+    res = 48
+    cam = frust_cam(res, eye=(0.1, -0.2, 3.0))
+    t, h, _, _ = queries.cast_rays_frustum((f,), (p,), cam, opts)
+    t, h = t.cpu().numpy(), h.cpu().numpy()
+    eye, look, up, left = cam[:4]
+    xs, ys = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    f32 = np.float32
+    tx = (f32(2) * xs.reshape(-1).astype(f32) / f32(res + 1) - f32(1)).astype(f32)
+    ty = (f32(2) * ys.reshape(-1).astype(f32) / f32(res + 1) - f32(1)).astype(f32)
+    tan = f32(np.tan(np.deg2rad(f32(60.0)) / f32(2)))
+    dirs = ofr.camera_ray(look, up, left, tan, tan, tx, ty).astype(np.float64)
+    t_star = ((0.25 - eye.astype(np.float64) @ nrm) / (dirs @ nrm)).reshape(res, res)
+    hit = (t_star > 0) & (t_star < 6.0)
+    assert np.all(h[hit] == 1) and np.all(h[~hit] == 0)
+    assert np.all(np.abs(t[hit] - t_star[hit]) <= 1.01e-3)
+
+
+# ------------------------------------------------------------ intersection
+@pytest.mark.parametrize("seeds", [(11, 12), (13, 14), (21, 22)])
+def test_find_any_intersection_matches_oracle(seeds):
+    """find_any_intersection (src/kd_tree.py:402-655): the same found flag and
+    the same point (the first certifying node in level order) as the oracle."""
+    note(f"[intersection] seeds {seeds}")
+    a = models.random_mlp([32, 32], seed=seeds[0])
+    b = models.random_mlp([48, 24], activation="tanh", seed=seeds[1])
+    fa, _ = implicit_mlp_utils.implicit_from_params(a, "affine_fixed")
+    fb, _ = implicit_mlp_utils.implicit_from_params(b, "affine_fixed")
+    found, ia, ib, loc = kd_tree.find_any_intersection((fa, fb), (a, b), LO, UP, 0.02)
+    rf, _, _, rloc, rn = oisect.find_any_intersection(ofuncs.OracleFunc(a), ofuncs.OracleFunc(b), LO, UP, 0.02)
+    note(f"  gpu {found} {loc.tolist()} ({kd_tree.find_any_intersection.last_n_processed} nodes); "
+         f"oracle {rf} {rloc.tolist()} ({rn} nodes)")
+    assert found == rf
+    assert np.array_equal(loc.numpy(), rloc)
+
+
+def test_find_any_intersection_touching_half_spaces():
+    """Half-spaces n.x < 0.2 and n.x > 0.2 touch on a plane: the search must
+    converge there through the near-surface test (exact affine bounds)."""
+    note("[intersection planes]")
+    nrm = np.array([0.3, 0.4, 0.86], np.float64)
+    nrm /= np.linalg.norm(nrm)
+
+    def plane(nv, c):
+        A1 = np.concatenate([np.eye(3), -np.eye(3)], axis=1)
+        A2 = np.concatenate([0.5 * nv, -0.5 * nv])[:, None]
+        return nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-c]),
+                                nmlp.squeeze_last()])
+    pa, pb = plane(nrm, 0.2), plane(-nrm, -0.2)
+    fa, _ = implicit_mlp_utils.implicit_from_params(pa, "affine_fixed")
+    fb, _ = implicit_mlp_utils.implicit_from_params(pb, "affine_fixed")
+    found, _, _, loc = kd_tree.find_any_intersection((fa, fb), (pa, pb), LO, UP, 0.05)
+    assert found and abs(float(loc.numpy().astype(np.float64) @ nrm) - 0.2) <= 0.05
+    pc = plane(-nrm, -0.5)
+    fc, _ = implicit_mlp_utils.implicit_from_params(pc, "affine_fixed")
+    found2, _, _, loc2 = kd_tree.find_any_intersection((fa, fc), (pa, pc), LO, UP, 0.05)
+    assert not found2 and np.all(loc2.numpy() == -777.0)
+
+
 # ------------------------------------------------------------ precision-mode rays
 def test_bf16_rays_plane_closed_form():
     """Precision-mode cast_rays on an exact plane (activation-free MLP): every
@@ -279,7 +380,7 @@ def test_bf16_rays_plane_closed_form():
     assert np.all(np.abs(t[hit] - t_star[hit]) <= 1.01e-3)
 
 
-@pytest.mark.parametrize("name", ["c1_relu_3x32", "small_tanh", "c2_relu_256x6", "fox"])
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "small_tanh", "fox", "bunny"])
 def test_bf16_rays_agree_with_fp32(name):
     """Precision-mode rays step on rigorous bounds, so their step sequence
     differs from the plain form's; both stop at the first sign change within
@@ -303,6 +404,29 @@ def test_bf16_rays_agree_with_fp32(name):
          f"steps {int(c32.sum().item())} / {int(c16.sum().item())}")
     assert same.mean() >= 0.98
     assert close.size == 0 or close.mean() >= 0.98
+
+
+def test_bf16_rays_never_invent_hits_on_deep_random_net():
+    """On the He-init 3->256x6->1 net the a-priori widening (which does not
+    shrink with the box) exceeds the function's scale, so the precision mode
+    certifies no step and its rays inch forward by hit_eps until n_max_step;
+    the plain form, not rigorous, steps on.  What must hold: every hit the
+    precision mode reports is a sign change the plain form also sees on that
+    ray, for the same function, within hit_eps."""
+    note("[bf16 rays] c2_relu_256x6 (deep random)")
+    p = model("c2_relu_256x6")
+    f32, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    f16, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision="bf16")
+    roots, dirs = camera(32)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    t32, h32, _, _ = queries.cast_rays((f32,), (p,), roots, dirs, opts)
+    t16, h16, c16, _ = queries.cast_rays((f16,), (p,), roots, dirs, opts)
+    t32, h32, t16, h16 = t32.cpu().numpy(), h32.cpu().numpy(), t16.cpu().numpy(), h16.cpu().numpy()
+    note(f"  hits fp32 {int((h32 > 0).sum())}, bf16 {int((h16 > 0).sum())}; bf16 steps {int(c16.sum().item())}")
+    hit16 = h16 > 0
+    assert np.all(h32[hit16] == h16[hit16])
+    assert np.all(np.abs(t16[hit16] - t32[hit16]) <= 1.001e-3)
 
 
 # ------------------------------------------------------------ precision mode

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import funcs as ofuncs, tree as otree, rays as orays, closest as ocp, mc as omc
+from oracle import frustum as ofr, intersect as oisect
 from oracle import affine as oaff, mlp as omlp
 from niq import mlp as nmlp, models
 
@@ -138,6 +139,113 @@ def test_rays_never_skip_a_sign_change():
     assert np.all(np.sign(v[:, :-1]) == np.sign(v0)[:, None]) or np.all(
         (np.sign(v[:, :-2]) == np.sign(v0)[:, None]).all(axis=1) | (hid > 0))
     assert np.all(cnt <= opts["n_max_step"])
+
+
+# ------------------------------------------------------------- frustum rays
+def pixel_rays(eye, look, up, left, fov, res):
+    """The sample ray of every pixel [x, y] of the frustum caster: a
+    single-pixel frustum's corners coincide at 2 x / (res + 1) - 1."""
+    xs, ys = np.meshgrid(np.arange(res), np.arange(res), indexing="ij")
+    f32 = np.float32
+    tx = (f32(2) * xs.reshape(-1).astype(f32) / f32(res + 1) - f32(1)).astype(f32)
+    ty = (f32(2) * ys.reshape(-1).astype(f32) / f32(res + 1) - f32(1)).astype(f32)
+    tan = f32(np.tan(np.deg2rad(f32(fov)) / f32(2)))
+    dirs = ofr.camera_ray(look, up, left, tan, tan, tx, ty)
+    return np.tile(eye, (dirs.shape[0], 1)), dirs
+
+
+def test_frustum_hits_plane_in_closed_form():
+    nrm = np.array([0.2, 0.3, 1.0], np.float32)
+    nrm /= np.linalg.norm(nrm)
+    c = 0.25
+    f = ofuncs.OracleFunc(plane_mlp(nrm, c), "affine_fixed")
+    eye = np.array([0.1, -0.2, 3.0], np.float32)
+    look, up, left = orays.look_at(eye)
+    opts = orays.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    opts["n_side_init"] = 8
+    res = 32
+    t, hid, cnt, ne = ofr.cast_rays_frustum((f,), eye, look, up, left, 60.0, 60.0, res, res, opts)
+    roots, dirs = pixel_rays(eye, look, up, left, 60.0, res)
+    t_star = ((c - roots @ nrm) / (dirs @ nrm)).reshape(res, res)
+    hit = (t_star > 0) & (t_star < 6.0)
+    assert np.all(hid[hit] == 1) and np.all(hid[~hit] == 0)
+    assert np.all(np.abs(t[hit] - t_star[hit]) <= 1.01e-3)
+    assert ne >= 64
+
+
+def test_frustum_matches_single_rays():
+    """Every pixel's frustum result equals the certified march of its own
+    sample ray (the ray oracle, pinned above): same hit function, t within
+    hit_eps (both stop inside the first sign change's hit_eps window)."""
+    p = models.config_model("c1_relu_3x32")
+    f = ofuncs.OracleFunc(p, "affine_fixed")
+    eye = np.array([0., 0., 3.], np.float32)
+    look, up, left = orays.look_at(eye)
+    opts = orays.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    opts["n_side_init"] = 8
+    res = 32
+    t, hid, cnt, ne = ofr.cast_rays_frustum((f,), eye, look, up, left, 60.0, 60.0, res, res, opts)
+    roots, dirs = pixel_rays(eye, look, up, left, 60.0, res)
+    rt, rh, _, _ = orays.cast_rays((f,), roots, dirs, opts)
+    rt, rh = rt.reshape(res, res), rh.reshape(res, res)
+    assert np.mean(rh == hid) >= 0.99
+    both = (rh == hid) & (hid > 0)
+    assert np.mean(np.abs(rt[both] - t[both]) <= 1.001e-3) >= 0.99
+
+
+def test_frustum_subdivision_partitions_the_image():
+    rng = np.array([[0, 0, 7, 4], [3, 5, 4, 9], [2, 2, 6, 6]], np.int32)
+    A, B = ofr.subdivide(rng)
+    for r, a, b in zip(rng, A, B):
+        area = (r[2] - r[0]) * (r[3] - r[1])
+        assert (a[2] - a[0]) * (a[3] - a[1]) + (b[2] - b[0]) * (b[3] - b[1]) == area
+        assert a[0] == r[0] and a[1] == r[1] and b[2] == r[2] and b[3] == r[3]
+    # the wider index axis is split (x on ties)
+    assert A[0][2] == 3 and B[0][0] == 3 and A[1][3] == 7 and A[2][2] == 4
+
+
+# --------------------------------------------------------- intersection
+def test_intersection_of_half_spaces():
+    """Two half-spaces f_A = n.x - a, f_B = -n.x - b (inside where negative)
+    overlap iff a + b > 0; with exact (activation-free) bounds the search
+    finds a point inside both exactly when they overlap."""
+    nrm = np.array([0.3, 0.4, 0.86], np.float32)
+    nrm /= np.linalg.norm(nrm)
+    lo, up = -np.ones(3, np.float32), np.ones(3, np.float32)
+    fa = ofuncs.OracleFunc(plane_mlp(nrm, 0.2), "affine_fixed")
+    fb = ofuncs.OracleFunc(plane_mlp(-nrm, 0.1), "affine_fixed")
+    found, ia, ib, loc, n = oisect.find_any_intersection(fa, fb, lo, up, 0.01)
+    assert found and (ia, ib) == (1, 2)
+    assert fa(loc[None])[0] < 0 and fb(loc[None])[0] < 0
+    fb2 = ofuncs.OracleFunc(plane_mlp(-nrm, -0.5), "affine_fixed")   # n.x > 0.5: disjoint from n.x < 0.2
+    found2, _, _, loc2, n2 = oisect.find_any_intersection(fa, fb2, lo, up, 0.01)
+    assert not found2 and np.all(loc2 == -777.0)
+    # touching half-spaces (n.x < 0.2 and n.x > 0.2): no sample is inside both,
+    # the search converges on the shared plane through the near-surface test
+    fb3 = ofuncs.OracleFunc(plane_mlp(-nrm, -0.2), "affine_fixed")
+    found3, _, _, loc3, n3 = oisect.find_any_intersection(fa, fb3, lo, up, 0.05)
+    assert found3 and abs(float(loc3 @ nrm) - 0.2) <= 0.05
+
+
+def test_intersection_point_is_inside_or_near_both():
+    a = models.random_mlp([32, 32], seed=11)
+    b = models.random_mlp([32, 32], seed=12)
+    fa, fb = ofuncs.OracleFunc(a), ofuncs.OracleFunc(b)
+    lo, up = -np.ones(3, np.float32), np.ones(3, np.float32)
+    eps = 0.02
+    found, _, _, loc, n = oisect.find_any_intersection(fa, fb, lo, up, eps)
+    if found:
+        # a sample inside both is preferred; these nets overlap on a large set
+        va, vb = fa(loc[None])[0], fb(loc[None])[0]
+        assert va < 0 and vb < 0
+    # brute force: a grid point inside both exists only if the search found one
+    g = np.linspace(-1, 1, 33, dtype=np.float32)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    both = (fa(pts) < 0) & (fb(pts) < 0)
+    if both.any():
+        assert found
 
 
 # ----------------------------------------------------------- closest point
