@@ -239,6 +239,52 @@ niq_status niq_mc_extract(const niq_model* m, const float* leaf_lower, const flo
  * -1 padded (host only; for checking). */
 int niq_mc_table(signed char* out);
 
+/* ---- level-set sampling and bulk properties ------------------------------------
+ * Replace the sampling rounds of sample_surface / sample_surface_uniform
+ * (src/kd_tree.py:220-336) and the mass integral of bulk_properties
+ * (src/kd_tree.py:804-863).  Candidates come from a counter-based stream,
+ * Philox4x32-10 keyed by `seed` (jax.random's threefry cannot be reproduced
+ * without JAX): candidate j uses counter (j mod 2^32, j div 2^32, tag, 0) with
+ * tag 0x53555246 (surface) or 0x4d415353 (mass); its node is
+ * floor(w0 * n_nodes / 2^32) and its position lower + u * (upper - lower) with
+ * u_k = (w_{k+1} >> 8) * 2^-24, in float32.  Node lists are (n_nodes, 3)
+ * float32 device arrays, 1 <= n_nodes < 2^31. */
+size_t niq_sample_workspace_bytes(int64_t n_per_round);
+
+/* Surface samples: candidates first_candidate + r * n_per_round + i of round r
+ * are accepted where |f| < width and appended in candidate order to
+ * out_points (n_samples, 3) from row *n_found on; *n_found is updated.  Rounds
+ * run until n_samples are found (the result is then the first accepted
+ * candidates of the stream, whatever n_per_round is) or max_rounds have run;
+ * compare *n_found with n_samples to tell which.  *n_rounds receives the
+ * rounds run.  Synchronising (once per round). */
+niq_status niq_sample_surface_nodes(const niq_model* m, const float* node_lower, const float* node_upper,
+                                    int64_t n_nodes, float width, uint64_t seed,
+                                    int64_t first_candidate, int64_t n_per_round, int64_t max_rounds,
+                                    float* out_points, int64_t n_samples, int64_t* n_found,
+                                    int64_t* n_rounds, void* workspace, size_t workspace_bytes,
+                                    void* stream);
+
+/* Sums of bulk_properties in float64: the volume and first moment of the
+ * interior boxes (src/kd_tree.py:841-853), the volume of the unknown boxes, and
+ * for n_samples mass candidates drawn from the unknown boxes the count with
+ * f < 0 and the sum of their positions (bulk_properties_sample_mass,
+ * src/kd_tree.py:804-833).  Either list may be empty (n = 0); with no unknown
+ * boxes no candidates are drawn.  Deterministic for a given device.
+ * Synchronising. */
+typedef struct niq_bulk_sums {
+  double interior_volume;
+  double interior_moment[3];
+  double unknown_volume;
+  double n_inside;
+  double inside_moment[3];
+} niq_bulk_sums;
+
+niq_status niq_bulk_sums_nodes(const niq_model* m, const float* unknown_lower, const float* unknown_upper,
+                               int64_t n_unknown, const float* interior_lower, const float* interior_upper,
+                               int64_t n_interior, int64_t n_samples, uint64_t seed, niq_bulk_sums* out,
+                               void* stream);
+
 /* ---- serving loop ------------------------------------------------------------
  * No counterpart in the reference (it runs one query at a time from one Python
  * thread, src/queries.py:134, src/kd_tree.py:103,766).  A server owns

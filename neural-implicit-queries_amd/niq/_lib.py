@@ -27,7 +27,8 @@ EXPORTS = ("niq_model_create", "niq_model_destroy", "niq_model_device_bytes",
            "niq_classify_boxes", "niq_classify_general_boxes", "niq_eval_points",
            "niq_tree_workspace_bytes", "niq_tree_build", "niq_tree_enqueue",
            "niq_tree_counters_bytes", "niq_tree_decode", "niq_cast_workspace_bytes",
-           "niq_cast_rays", "niq_cast_rays_frustum", "niq_find_any_intersection", "niq_closest_point", "niq_mc_extract", "niq_mc_table",
+           "niq_cast_rays", "niq_cast_rays_frustum", "niq_find_any_intersection", "niq_closest_point",
+           "niq_sample_workspace_bytes", "niq_sample_surface_nodes", "niq_bulk_sums_nodes", "niq_mc_extract", "niq_mc_table",
            "niq_serve_create", "niq_serve_destroy", "niq_serve_submit_rays", "niq_serve_submit_tree",
            "niq_serve_submit_closest_point", "niq_serve_poll", "niq_serve_bytes_in_use",
            "niq_last_error", "niq_version")
@@ -95,6 +96,12 @@ class ServeInfo(ctypes.Structure):
                 ("graph_replay", ctypes.c_int32), ("n_processed", ctypes.c_int64)]
 
 
+class BulkSums(ctypes.Structure):
+    _fields_ = [("interior_volume", ctypes.c_double), ("interior_moment", ctypes.c_double * 3),
+                ("unknown_volume", ctypes.c_double), ("n_inside", ctypes.c_double),
+                ("inside_moment", ctypes.c_double * 3)]
+
+
 _lib = None
 
 
@@ -127,6 +134,10 @@ def load_library():
                                           ctypes.POINTER(I64), P]),
         "niq_mc_extract": (I, [P, P, P, I64, I, P, I64, ctypes.POINTER(I64), P]),
         "niq_mc_table": (I, [P]),
+        "niq_sample_workspace_bytes": (S, [I64]),
+        "niq_sample_surface_nodes": (I, [P, P, P, I64, F, ctypes.c_uint64, I64, I64, I64, P, I64,
+                                         ctypes.POINTER(I64), ctypes.POINTER(I64), P, S, P]),
+        "niq_bulk_sums_nodes": (I, [P, P, P, I64, P, P, I64, I64, ctypes.c_uint64, P, P]),
         "niq_serve_create": (I, [I, S, I64, ctypes.POINTER(P)]),
         "niq_serve_destroy": (I, [P]),
         "niq_serve_submit_rays": (I, [P, P, I, P, P, P, I64, P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]),

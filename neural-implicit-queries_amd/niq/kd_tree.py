@@ -205,6 +205,151 @@ def find_any_intersection(func_tuple, params_tuple, lower, upper, eps, viz_nodes
     return False, 0, 0, torch.tensor([-777., -777., -777.], dtype=torch.float32)
 
 
+def _seed(rngkey):
+    """A 64-bit seed from an int or a two-word key laid out like
+    jax.random.PRNGKey's ([high word, low word])."""
+    if isinstance(rngkey, int):
+        return rngkey & 0xFFFFFFFFFFFFFFFF
+    words = [int(v) & 0xFFFFFFFF for v in torch.as_tensor(rngkey).reshape(-1).tolist()]
+    if len(words) != 2:
+        raise ValueError("rngkey must be an int or a pair of 32-bit words")
+    return (words[0] << 32) | words[1]
+
+
+def _valid_nodes(node_valid, node_lower, node_upper):
+    """The valid nodes come first (src/kd_tree.py:225-227)."""
+    n = int(torch.as_tensor(node_valid).sum().item())
+    lo = torch.as_tensor(node_lower).to(device="cuda", dtype=torch.float32)[:n].contiguous()
+    up = torch.as_tensor(node_upper).to(device="cuda", dtype=torch.float32)[:n].contiguous()
+    return n, lo, up
+
+
+def sample_nodes(params, node_lower, node_upper, n_nodes, n_samples, width, rngkey, n_samples_per_round,
+                 found_sample_points=None, found_start_ind=0, first_candidate=0, max_rounds=100000):
+    """Rounds of sample_surface_iter (src/kd_tree.py:220-246) until n_samples
+    points with |f| < width are found: returns (points (n_samples, 3), found,
+    rounds).  Candidates are drawn from the Philox stream of
+    niq_sample_surface_nodes; the points are the first accepted candidates of
+    that stream whatever the round size."""
+    _lib.require_gpu()
+    lib = _lib.load_library()
+    m = _mlp.device_model(params)
+    if n_nodes < 1:
+        raise ValueError("no nodes to sample from")
+    out = found_sample_points
+    if out is None:
+        out = torch.zeros((n_samples, 3), dtype=torch.float32, device="cuda")
+    if out.shape != (n_samples, 3) or out.dtype != torch.float32 or not out.is_cuda or not out.is_contiguous():
+        raise ValueError("found_sample_points must be a contiguous (n_samples, 3) float32 device tensor")
+    ws = torch.empty(lib.niq_sample_workspace_bytes(int(n_samples_per_round)), dtype=torch.uint8, device="cuda")
+    found = ctypes.c_int64(int(found_start_ind))
+    rounds = ctypes.c_int64(0)
+    _lib.check(lib.niq_sample_surface_nodes(m.handle, _lib.ptr(node_lower), _lib.ptr(node_upper), int(n_nodes),
+                                            float(width), _seed(rngkey), int(first_candidate),
+                                            int(n_samples_per_round), int(max_rounds), _lib.ptr(out),
+                                            int(n_samples), ctypes.byref(found), ctypes.byref(rounds),
+                                            _lib.ptr(ws), ws.numel(), _lib.stream_handle()))
+    return out, int(found.value), int(rounds.value)
+
+
+def _sample_all(params, lo, up, n, n_samples, width, rngkey, per_round, max_rounds=100000):
+    """The reference's loop runs until the output is full (src/kd_tree.py:260-270);
+    this one gives up after max_rounds rounds."""
+    pts, found, rounds = sample_nodes(params, lo, up, n, n_samples, width, rngkey, per_round, max_rounds=max_rounds)
+    if found < n_samples:
+        raise _lib.CapacityError(_lib.NIQ_ERR_CAPACITY,
+                                 f"found {found} of {n_samples} samples with |f| < {width} in {rounds} rounds")
+    return pts, rounds
+
+
+def sample_surface_iter(func, params, n_samples_per_round, width, rngkey, u_node_valid, u_node_lower, u_node_upper,
+                        found_sample_points, found_start_ind):
+    """One round (src/kd_tree.py:220-246): (found_sample_points, found_start_ind).
+    Like the reference, the same key draws the same candidates; pass a new key
+    per round."""
+    n, lo, up = _valid_nodes(u_node_valid, u_node_lower, u_node_upper)
+    pts, found, _ = sample_nodes(params, lo, up, n, found_sample_points.shape[0], width, rngkey,
+                                 n_samples_per_round, found_sample_points, int(found_start_ind), max_rounds=1)
+    return pts, found
+
+
+def _round_size(n_samples, per_sample):
+    # Any round size gives the same points; large rounds keep the chip busy.
+    return max(1024, min(per_sample * n_samples, 1 << 22))
+
+
+def sample_surface(func, params, lower, upper, n_samples, width, rngkey, n_node_thresh=4096):
+    """src/kd_tree.py:249-272: a tree over |f| <= width (offset=width), then
+    rejection sampling from its unknown nodes.  Returns (n_samples, 3)."""
+    out = construct_uniform_unknown_levelset_tree(func, params, lower, upper, node_terminate_thresh=n_node_thresh,
+                                                  offset=width)
+    n, lo, up = _valid_nodes(out["unknown_node_valid"], out["unknown_node_lower"], out["unknown_node_upper"])
+    pts, rounds = _sample_all(params, lo, up, n, n_samples, width, rngkey, _round_size(n_samples, 3))
+    sample_surface.last_rounds = rounds
+    return pts
+
+
+def sample_surface_uniform_iter(func, params, n_samples_per_round, width, rngkey, lower, upper,
+                                found_sample_points, found_start_ind):
+    """One round over the whole box (src/kd_tree.py:276-301)."""
+    lo = torch.as_tensor(lower).to(device="cuda", dtype=torch.float32).reshape(1, 3).contiguous()
+    up = torch.as_tensor(upper).to(device="cuda", dtype=torch.float32).reshape(1, 3).contiguous()
+    pts, found, _ = sample_nodes(params, lo, up, 1, found_sample_points.shape[0], width, rngkey,
+                                 n_samples_per_round, found_sample_points, int(found_start_ind), max_rounds=1)
+    return pts, found
+
+
+def sample_surface_uniform(func, params, lower, upper, n_samples, width, rngkey):
+    """src/kd_tree.py:321-336: rejection sampling over the whole box."""
+    lo = torch.as_tensor(lower).to(device="cuda", dtype=torch.float32).reshape(1, 3).contiguous()
+    up = torch.as_tensor(upper).to(device="cuda", dtype=torch.float32).reshape(1, 3).contiguous()
+    pts, rounds = _sample_all(params, lo, up, 1, n_samples, width, rngkey, _round_size(n_samples, 10))
+    sample_surface_uniform.last_rounds = rounds
+    return pts
+
+
+def bulk_sums(params, unknown_lower, unknown_upper, n_unknown, interior_lower, interior_upper, n_interior,
+              n_samples, rngkey):
+    """niq_bulk_sums_nodes: the float64 sums behind bulk_properties."""
+    _lib.require_gpu()
+    lib = _lib.load_library()
+    m = _mlp.device_model(params)
+    out = _lib.BulkSums()
+    _lib.check(lib.niq_bulk_sums_nodes(m.handle, _lib.ptr(unknown_lower) if n_unknown else None,
+                                       _lib.ptr(unknown_upper) if n_unknown else None, int(n_unknown),
+                                       _lib.ptr(interior_lower) if n_interior else None,
+                                       _lib.ptr(interior_upper) if n_interior else None, int(n_interior),
+                                       int(n_samples), _seed(rngkey), ctypes.byref(out), _lib.stream_handle()))
+    return out
+
+
+def bulk_properties_sample_mass(func, params, node_valid, node_lower, node_upper, n_samples, rngkey):
+    """src/kd_tree.py:804-833: (mass, centroid moment (3,)) of the samples
+    inside (f < 0) drawn from the valid nodes; the centroid is not divided by
+    the mass, as in the reference."""
+    n, lo, up = _valid_nodes(node_valid, node_lower, node_upper)
+    s = bulk_sums(params, lo, up, n, None, None, 0, n_samples, rngkey)
+    vps = s.unknown_volume / n_samples if n_samples > 0 else 0.0
+    mass = torch.tensor(vps * s.n_inside, dtype=torch.float32)
+    centroid = torch.tensor([vps * v for v in s.inside_moment], dtype=torch.float32)
+    return mass, centroid
+
+
+def bulk_properties(func, params, lower, upper, rngkey, n_expand=int(1e4), n_sample=int(1e6)):
+    """src/kd_tree.py:835-863: (mass, centroid (3,)) of the region f < 0:
+    interior boxes exactly, unknown boxes by n_sample mass samples."""
+    out = construct_uniform_unknown_levelset_tree(func, params, lower, upper, with_interior_nodes=True,
+                                                  node_terminate_thresh=n_expand)
+    nu, ulo, uup = _valid_nodes(out["unknown_node_valid"], out["unknown_node_lower"], out["unknown_node_upper"])
+    ni, ilo, iup = _valid_nodes(out["interior_node_valid"], out["interior_node_lower"], out["interior_node_upper"])
+    s = bulk_sums(params, ulo, uup, nu, ilo, iup, ni, n_sample, rngkey)
+    vps = s.unknown_volume / n_sample if n_sample > 0 else 0.0
+    mass = s.interior_volume + vps * s.n_inside
+    # an empty region gives NaN like the reference's 0 / 0
+    cen = [(s.interior_moment[k] + vps * s.inside_moment[k]) / mass if mass else float("nan") for k in range(3)]
+    return torch.tensor(mass, dtype=torch.float32), torch.tensor(cen, dtype=torch.float32)
+
+
 def extract_cells(params, node_lower, node_upper, n_leaves, n_subcell_depth):
     """Marching cubes over the first n_leaves boxes: (T, 3, 3) triangles."""
     lib = _lib.load_library()

@@ -25,7 +25,7 @@ import pytest
 import torch
 
 from oracle import funcs as ofuncs, tree as otree, rays as orays, mlp as omlp, affine as oaff
-from oracle import frustum as ofr, intersect as oisect
+from oracle import frustum as ofr, intersect as oisect, sampling as osamp
 from niq import models, mlp as nmlp, implicit_mlp_utils, kd_tree, queries, render
 
 pytestmark = pytest.mark.gpu
@@ -355,6 +355,114 @@ def test_find_any_intersection_touching_half_spaces():
     fc, _ = implicit_mlp_utils.implicit_from_params(pc, "affine_fixed")
     found2, _, _, loc2 = kd_tree.find_any_intersection((fa, fc), (pa, pc), LO, UP, 0.05)
     assert not found2 and np.all(loc2.numpy() == -777.0)
+
+
+# ------------------------------------------------------------ sampling
+def gpu_nodes(func, p, **kw):
+    out = kd_tree.construct_uniform_unknown_levelset_tree(func, p, LO, UP, **kw)
+    res = {}
+    for key in ("unknown", "interior"):
+        if f"{key}_node_valid" in out:
+            n = int(out[f"{key}_node_valid"].sum().item())
+            res[key] = (n, out[f"{key}_node_lower"][:n].contiguous(), out[f"{key}_node_upper"][:n].contiguous())
+    return res
+
+
+@pytest.mark.parametrize("name,width", [("c1_relu_3x32", 0.02), ("small_tanh", 0.01), ("fox", 0.005)])
+def test_sample_surface_matches_oracle(name, width):
+    """sample_surface rounds (src/kd_tree.py:220-272) on the device tree's
+    nodes: the same points as the oracle's stream (bit for bit) unless a
+    consumed candidate's |f| lies within 1e-5 * scale of width; any round
+    size and a stream offset give the same points."""
+    note(f"[sample surface] {name}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    n, lo, up = gpu_nodes(func, p, node_terminate_thresh=2048, offset=width)["unknown"]
+    assert n > 0
+    ns = 3000
+    got, found, rounds = kd_tree.sample_nodes(p, lo, up, n, ns, width, 99, 4096)
+    got = got.cpu().numpy()
+    ref, r_rounds, vals = osamp.sample_nodes(ofuncs.OracleFunc(p), lo.cpu().numpy(), up.cpu().numpy(), n, ns,
+                                             width, 99, 4096)
+    scale = float(np.max(np.abs(vals)))
+    near = int(np.sum(np.abs(np.abs(vals) - width) <= 1e-5 * scale))
+    same = int(np.sum(np.all(got == ref, axis=1)))
+    note(f"  {n} nodes; rounds {rounds} / {r_rounds}; identical rows {same} / {ns}; near-threshold candidates {near}")
+    assert found == ns
+    if near == 0:
+        assert np.array_equal(got, ref) and rounds == r_rounds
+    else:
+        fv = omlp.eval_points(p, got, np.float32).reshape(-1)
+        assert np.all(np.abs(fv) < width + 1e-5 * scale)
+    # other round sizes and a resumed stream give the same points
+    got2, _, rounds2 = kd_tree.sample_nodes(p, lo, up, n, ns, width, 99, 1 << 20)
+    assert np.array_equal(got2.cpu().numpy(), got) and rounds2 == 1
+    half = torch.zeros((ns, 3), dtype=torch.float32, device="cuda")
+    part, f1, _ = kd_tree.sample_nodes(p, lo, up, n, ns, width, 99, 1000, half, 0, max_rounds=1)
+    assert 0 < f1 < ns and np.array_equal(part[:f1].cpu().numpy(), got[:f1])
+
+
+def test_sample_surface_plane_closed_form():
+    """Samples of a plane lie in its |f| < width slab and are uniform there."""
+    note("[sample surface plane]")
+    c, w = 0.3, 0.02
+    A1 = np.concatenate([np.eye(3), -np.eye(3)], axis=1)
+    A2 = np.concatenate([0.5 * np.array([1.0, 0, 0]), -0.5 * np.array([1.0, 0, 0])])[:, None]
+    pl = nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-c]), nmlp.squeeze_last()])
+    func, _ = implicit_mlp_utils.implicit_from_params(pl, "affine_fixed")
+    pts = kd_tree.sample_surface(func, pl, LO, UP, 200000, w, 5).cpu().numpy()
+    assert np.all(np.abs(pts[:, 0] - c) < w + 1e-6)
+    assert abs(pts[:, 0].mean() - c) < 0.01 * w
+    assert abs(pts[:, 0].var() - (2 * w) ** 2 / 12) < 0.02 * (2 * w) ** 2 / 12
+    assert np.all(np.abs(pts[:, 1:].mean(0)) < 0.01) and np.all(np.abs(pts[:, 1:].var(0) - 1 / 3) < 0.005)
+    pu = kd_tree.sample_surface_uniform(func, pl, LO, UP, 5000, w, 6).cpu().numpy()
+    assert np.all(np.abs(pu[:, 0] - c) < w + 1e-6)
+
+
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "fox"])
+def test_bulk_sums_match_oracle(name):
+    """bulk_properties sums (src/kd_tree.py:804-863) on the device tree's
+    nodes: box volumes and moments to 1e-12, and the same inside count as the
+    oracle's stream up to its near-zero candidates."""
+    note(f"[bulk sums] {name}")
+    p = model(name)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    nodes = gpu_nodes(func, p, node_terminate_thresh=4000, with_interior_nodes=True)
+    nu, ul, uu = nodes["unknown"]
+    ni, il, iu = nodes["interior"]
+    ns = 200000
+    s = kd_tree.bulk_sums(p, ul, uu, nu, il, iu, ni, ns, 31)
+    m_int, c_int, n_in, mom = osamp.bulk_sums(ofuncs.OracleFunc(p), ul.cpu().numpy(), uu.cpu().numpy(), nu,
+                                              il.cpu().numpy(), iu.cpu().numpy(), ni, ns, 31)
+    pos, _ = osamp.candidates(31, osamp.TAG_MASS, 0, ns, ul.cpu().numpy(), uu.cpu().numpy(), nu)
+    v = omlp.eval_points(p, pos, np.float32).reshape(-1)
+    near = int(np.sum(np.abs(v) <= 1e-5 * float(np.max(np.abs(v)))))
+    note(f"  unknown {nu}, interior {ni}; inside {s.n_inside:.0f} / {n_in}; near-zero candidates {near}")
+    assert abs(s.interior_volume - m_int) <= 1e-12 * max(1.0, m_int)
+    assert np.allclose(list(s.interior_moment), c_int, rtol=1e-12, atol=1e-12)
+    area = float(np.prod(uu.cpu().numpy().astype(np.float64) - ul.cpu().numpy(), axis=1).sum())
+    assert abs(s.unknown_volume - area) <= 1e-12 * max(1.0, area)
+    assert abs(s.n_inside - n_in) <= near
+    if near == 0:
+        assert np.allclose(list(s.inside_moment), mom, rtol=1e-12, atol=1e-9)
+
+
+def test_bulk_properties_half_space_closed_form():
+    """bulk_properties of {x < 0.3} in [-1, 1]^3: volume 5.2, centroid (-0.35, 0, 0)."""
+    note("[bulk properties plane]")
+    A1 = np.concatenate([np.eye(3), -np.eye(3)], axis=1)
+    A2 = np.concatenate([0.5 * np.array([1.0, 0, 0]), -0.5 * np.array([1.0, 0, 0])])[:, None]
+    pl = nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-0.3]),
+                          nmlp.squeeze_last()])
+    func, _ = implicit_mlp_utils.implicit_from_params(pl, "affine_fixed")
+    mass, cen = kd_tree.bulk_properties(func, pl, LO, UP, 3)
+    note(f"  mass {float(mass):.6f} centroid {cen.tolist()}")
+    assert abs(float(mass) - 5.2) < 1e-3
+    assert np.allclose(cen.numpy(), [-0.35, 0.0, 0.0], atol=5e-4)
+    m_b, c_b = kd_tree.bulk_properties_sample_mass(func, pl, *[torch.ones(1, dtype=torch.bool)],
+                                                   torch.tensor([[-1.0, -1, -1]]), torch.tensor([[1.0, 1, 1]]),
+                                                   400000, 4)
+    assert abs(float(m_b) - 5.2) < 0.03 and abs(float(c_b[0]) / float(m_b) + 0.35) < 0.01
 
 
 # ------------------------------------------------------------ precision-mode rays
