@@ -42,6 +42,11 @@ typedef int niq_status;
 #define NIQ_MODE_AFFINE_APPEND 3
 #define NIQ_MODE_AFFINE_ALL 4
 #define NIQ_MODE_SLOPE_INTERVAL 5
+/* weak-SDF Lipschitz test (src/sdf.py:17-50, the 'sdf' mode of
+ * src/implicit_mlp_utils.py:24-30): a box is UNKNOWN where
+ * |f(centre)| < sdf_lipschitz * radius, radius = sqrt(sum_v |vec_v|^2);
+ * otherwise POSITIVE if f(centre) > offset, NEGATIVE if f(centre) < -offset */
+#define NIQ_MODE_SDF 6
 
 /* arithmetic of the error-term contraction */
 #define NIQ_PREC_FP32 0
@@ -63,7 +68,8 @@ typedef struct niq_eval_cfg {
   int variant;         /* kernel variant, 0 = default; precision mode: 1 streams
                           the packed bf16 weight copies, 2 splits FP32 weights
                           in registers (the default) */
-  int reserved[2];
+  float sdf_lipschitz; /* NIQ_MODE_SDF: Lipschitz bound of f (sdf_lipschitz kwarg) */
+  int reserved;
 } niq_eval_cfg;
 
 /* ---- model ------------------------------------------------------------

@@ -54,13 +54,41 @@ niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long 
   return fail(NIQ_ERR_UNSUPPORTED, "width");
 }
 
-// Route a classification by (mode, precision, number of box vectors).
+// The weak-SDF test: one point plane, any number of box vectors.
+template <int W>
+niq_status launch_classify_sdf_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  constexpr int NB = TileCfg<W>::NBP, NW = TileCfg<W>::NW;
+  auto k = k_classify_sdf<NB, NW, W>;
+  const long long tiles = (max_n + NB - 1) / NB;
+  if (tiles == 0) return NIQ_OK;
+  const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+  hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, a);
+  HIPCHK(hipGetLastError());
+  return NIQ_OK;
+}
+
+niq_status launch_classify_sdf(const niq_model* m, const niq_eval_cfg* cfg, ClassifyArgs a, long long max_n,
+                               hipStream_t s) {
+  if (a.recheck_tol > 0.f) return fail(NIQ_ERR_UNSUPPORTED, "the sdf mode has no FP64 re-test");
+  a.lipschitz = cfg->sdf_lipschitz;
+  switch (m->wt) {
+    case 64: return launch_classify_sdf_w<64>(m, a, max_n, s);
+    case 128: return launch_classify_sdf_w<128>(m, a, max_n, s);
+    case 256: return launch_classify_sdf_w<256>(m, a, max_n, s);
+    case 512: return launch_classify_sdf_w<512>(m, a, max_n, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
 }  // namespace
 
 niq_status niqi::classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec,
                                    const ClassifyArgs& a, long long max_n, hipStream_t s) {
   const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
   const int prec = cfg ? cfg->precision : NIQ_PREC_FP32;
+  if (mode == NIQ_MODE_SDF) {
+    if (prec != NIQ_PREC_FP32) return fail(NIQ_ERR_UNSUPPORTED, "the sdf mode evaluates points in FP32 only");
+    return launch_classify_sdf(m, cfg, a, max_n, s);
+  }
   if (prec == NIQ_PREC_BF16) return classify_bf16(m, cfg, nvec, a, max_n, s);
   if (prec != NIQ_PREC_FP32) return fail(NIQ_ERR_INVALID, "unknown precision");
   if (mode == NIQ_MODE_AFFINE_ALL || mode == NIQ_MODE_AFFINE_TRUNCATE || mode == NIQ_MODE_AFFINE_APPEND) {

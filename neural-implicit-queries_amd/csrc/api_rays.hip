@@ -22,7 +22,7 @@ namespace {
 template <int KIND, int V, int W>
 niq_status launch_rays_w(const DevModel* models, const RayArgs& a, hipStream_t s) {
   constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
-  constexpr int R = Planes<KIND, V, 2>::R;
+  constexpr int R = Planes<KIND, V, KIND == 2 ? 3 : 2>::R;
   if constexpr (!Fits<R, NB, W>::value) {
     return fail(NIQ_ERR_UNSUPPORTED, "ray range form does not fit in LDS at this width");
   } else {
@@ -136,7 +136,10 @@ extern "C" niq_status niq_cast_rays(const niq_model* const* models, int n_models
     else if (mode == NIQ_MODE_AFFINE_FIXED) st = launch_rays<0, 1>(wt, table, a, s);
     else if (mode == NIQ_MODE_INTERVAL) st = launch_rays<0, 0>(wt, table, a, s);
     else if (mode == NIQ_MODE_SLOPE_INTERVAL) st = launch_rays<1, 1>(wt, table, a, s);
-    else return fail(NIQ_ERR_UNSUPPORTED, "cast_rays supports interval, affine_fixed and slope_interval");
+    else if (mode == NIQ_MODE_SDF) {
+      a.lipschitz = cfg->sdf_lipschitz;
+      st = launch_rays<2, 0>(wt, table, a, s);
+    } else return fail(NIQ_ERR_UNSUPPORTED, "cast_rays supports interval, affine_fixed, slope_interval and sdf");
     if (st) return st;
   }
   std::vector<int> h(o->n_max_step + 2, 0);

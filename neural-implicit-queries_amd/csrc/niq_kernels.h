@@ -45,7 +45,18 @@ struct ClassifyArgs {
   int recheck_cap;
   void* lw_buf;            // precision mode, layer by layer: activation scratch
   size_t lw_bytes;         //   (null: allocated per call, stream ordered)
+  float lipschitz;         // weak-SDF test: Lipschitz bound of f
 };
+
+// Weak-SDF sign code (src/sdf.py:42-48): UNKNOWN where the value at the centre
+// could change sign within the box, else its sign against the offset.
+__device__ __forceinline__ int sdf_code(float val, float rad, float lipschitz, float offset) {
+  const bool can_change = fabsf(val) - __fmul_rn(rad, lipschitz) < 0.f;
+  int code = CODE_UNKNOWN;
+  if (!can_change && val > offset) code = CODE_POSITIVE;
+  if (!can_change && val < -offset) code = CODE_NEGATIVE;
+  return code;
+}
 
 // Fill the layer-0 input planes of one tile (features 0..15, zero padded).
 template <int KIND, int V, int NB, int NT, int WMAX>
@@ -106,6 +117,69 @@ k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
           const int slot = atomicAdd(a.recheck_cnt, 1);
           if (slot < a.recheck_cap) a.recheck_idx[slot] = (int)i;
         }
+      }
+    });
+  }
+}
+
+// Weak-SDF classification (src/sdf.py:31-50): f at the box centre against
+// lipschitz * radius, radius = sqrt(sum_v |vec_v|^2), |vec| = sqrt(sum_k vec_k^2)
+// (axis boxes: vecs = diag(upper - centre), src/implicit_function.py:28-37).
+// bounds, when asked for, are val -/+ lipschitz * radius.
+template <int NB, int NW, int WMAX>
+__global__ void __launch_bounds__(NW * 64)
+k_classify_sdf(const DevModel* __restrict__ Mp, ClassifyArgs a) {
+  using PL = Planes<2, 0, 1>;
+  constexpr int NT = NW * 64;
+  __shared__ float X[NB * WMAX];
+  __shared__ float Rd[NB];
+  const long long n = a.n_dev ? (long long)(*a.n_dev) : a.n;
+  const long long ntiles = (n + NB - 1) / NB;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long i0 = tile * NB;
+    __syncthreads();  // the previous tile's epilogue has read Rd
+    for (int e = threadIdx.x; e < NB * 16; e += NT) {
+      const int b = e >> 4, k = e & 15;
+      const long long i = i0 + b;
+      float val = 0.f;
+      if (i < n && k < a.d)
+        val = a.lower ? 0.5f * (a.lower[i * a.d + k] + a.upper[i * a.d + k]) : a.center[i * a.d + k];
+      X[xoff<NB, WMAX>(0, b, k)] = val;
+    }
+    for (int b = threadIdx.x; b < NB; b += NT) {
+      const long long i = i0 + b;
+      float r2 = 0.f;
+      if (i < n) {
+        if (a.lower) {
+          for (int k = 0; k < a.d; ++k) {
+            const float c = 0.5f * (a.lower[i * a.d + k] + a.upper[i * a.d + k]);
+            const float h = a.upper[i * a.d + k] - c;
+            const float nv = sqrtf(__fmul_rn(h, h));
+            r2 = __fadd_rn(r2, __fmul_rn(nv, nv));
+          }
+        } else {
+          for (int v = 0; v < a.vin; ++v) {
+            float ss = 0.f;
+            for (int k = 0; k < a.d; ++k) {
+              const float x = a.vecs[(i * a.vin + v) * a.d + k];
+              ss = __fadd_rn(ss, __fmul_rn(x, x));
+            }
+            const float nv = sqrtf(ss);
+            r2 = __fadd_rn(r2, __fmul_rn(nv, nv));
+          }
+        }
+      }
+      Rd[b] = sqrtf(r2);
+    }
+    mlp_forward<PL, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+      const long long i = i0 + b;
+      if (i >= n) return;
+      const float val = v[0], rad = Rd[b];
+      a.codes[i] = sdf_code(val, rad, a.lipschitz, a.offset);
+      if (a.bounds) {
+        const float r = __fmul_rn(rad, a.lipschitz);
+        a.bounds[2 * i] = val - r;
+        a.bounds[2 * i + 1] = val + r;
       }
     });
   }
@@ -421,6 +495,7 @@ struct RayArgs {
   float* out_t; int* out_hit; int* out_count;
   int* next_ray;           // work counter, zero on entry
   int* iter_hist;          // (n_max_step + 2,) rays by iterations lived, nullable
+  float lipschitz;         // weak-SDF form (KIND 2): Lipschitz bound of f
 };
 
 // Persistent certified-step ray caster (src/queries.py:39-175).  A workgroup
@@ -429,15 +504,18 @@ struct RayArgs {
 // Terminated slots are refilled from a global counter, so the tile stays full
 // and no host round trip happens until all rays are done.  Slot state is
 // replicated in lanes 0..NB-1 of every wave (each wave computes the same
-// update), and per-slot results are exchanged through LDS.
+// update), and per-slot results are exchanged through LDS.  KIND 2 is the
+// weak-SDF form: three point planes (segment centre, start, start + hit_eps)
+// and the segment radius |half_vec| kept in LDS per slot.
 template <int KIND, int V, int NB, int NW, int WMAX>
 __global__ void __launch_bounds__(NW * 64)
 k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
-  using PL = Planes<KIND, V, 2>;
+  using PL = Planes<KIND, V, KIND == 2 ? 3 : 2>;
   constexpr int R = PL::R;
-  constexpr int RB = PL::RB;
+  constexpr int PT = KIND == 2 ? 1 : PL::RB;   // first hit-test point plane
   constexpr int NT = NW * 64;
   __shared__ float X[R * NB * WMAX];
+  __shared__ float Rd[KIND == 2 ? NB : 1];
   int* Xi = reinterpret_cast<int*>(X);
   const int lane = threadIdx.x & 63;
   const bool slot_lane = lane < NB;
@@ -482,6 +560,7 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
           const int b = threadIdx.x;
           const bool live = rid >= 0;
           const float hs = 0.5f * step;
+          float hh = 0.f;
           for (int k = 0; k < 3; ++k) {
             const float ps = ro[k] + t * di[k];
             const float hv = hs * di[k];
@@ -491,19 +570,30 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
             for (int p = 0; p < R; ++p) vals[p] = 0.f;
             vals[0] = pm;
             if constexpr (KIND == 0 && V == 0) vals[1] = fabsf(hv);
-            else vals[1] = hv;
-            vals[RB] = ps;
-            vals[RB + 1] = pe;
+            else if constexpr (KIND != 2) vals[1] = hv;
+            vals[PT] = ps;
+            vals[PT + 1] = pe;
+            hh = __fadd_rn(hh, __fmul_rn(hv, hv));
 #pragma unroll
             for (int p = 0; p < R; ++p) X[xoff<NB, WMAX>(p, b, k)] = live ? vals[p] : 0.f;
           }
+          if constexpr (KIND == 2) {
+            const float nv = sqrtf(hh);
+            Rd[b] = sqrtf(__fmul_rn(nv, nv));
+          }
         }
         mlp_forward<PL, KIND, V, NB, NW, WMAX>(models[f], X, [&](const float* v, int b) {
-          float lo, hi, scale;
-          final_bounds<KIND, V>(v, lo, hi, scale);
-          X[4 * b + 0] = __int_as_float(sign_code(lo, hi, 0.f));
-          X[4 * b + 1] = v[RB];
-          X[4 * b + 2] = v[RB + 1];
+          int code;
+          if constexpr (KIND == 2) {
+            code = sdf_code(v[0], Rd[b], a.lipschitz, 0.f);
+          } else {
+            float lo, hi, scale;
+            final_bounds<KIND, V>(v, lo, hi, scale);
+            code = sign_code(lo, hi, 0.f);
+          }
+          X[4 * b + 0] = __int_as_float(code);
+          X[4 * b + 1] = v[PT];
+          X[4 * b + 2] = v[PT + 1];
         });
         __syncthreads();
         if (slot_lane) {

@@ -20,7 +20,7 @@ NIQ_ERR_HIP = -4
 
 ACT = {"none": 0, "relu": 1, "elu": 2, "sin": 3, "tanh": 4}
 MODE = {"interval": 0, "affine_fixed": 1, "affine_truncate": 2, "affine_append": 3,
-        "affine_all": 4, "slope_interval": 5}
+        "affine_all": 4, "slope_interval": 5, "sdf": 6}
 PREC = {"fp32": 0, "bf16": 1}
 
 EXPORTS = ("niq_model_create", "niq_model_destroy", "niq_model_device_bytes",
@@ -48,7 +48,7 @@ class EvalCfg(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("precision", ctypes.c_int),
                 ("truncate_count", ctypes.c_int), ("n_append", ctypes.c_int),
                 ("recheck_tol", ctypes.c_float), ("variant", ctypes.c_int),
-                ("reserved", ctypes.c_int * 2)]
+                ("sdf_lipschitz", ctypes.c_float), ("reserved", ctypes.c_int)]
 
 
 class TreeParams(ctypes.Structure):
@@ -179,8 +179,9 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
-def eval_cfg(mode="affine_fixed", precision="fp32", truncate_count=0, n_append=0, recheck_tol=0.0, variant=0):
+def eval_cfg(mode="affine_fixed", precision="fp32", truncate_count=0, n_append=0, recheck_tol=0.0, variant=0,
+             sdf_lipschitz=1.0):
     if mode not in MODE:
         raise ValueError("invalid mode")
     return EvalCfg(MODE[mode], PREC[precision], int(truncate_count), int(n_append),
-                   float(recheck_tol), int(variant), (ctypes.c_int * 2)())
+                   float(recheck_tol), int(variant), float(sdf_lipschitz), 0)

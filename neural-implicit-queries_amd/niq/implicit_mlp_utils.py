@@ -5,14 +5,14 @@ npz dictionary (float32 NumPy arrays) and ``func`` runs on the gfx950 kernels.
 Extra keyword arguments of this build: ``precision`` ('fp32' | 'bf16') and
 ``recheck_tol`` (FP64 re-test band, relative to the bound's scale).
 """
-from . import mlp, affine, slope_interval
+from . import mlp, affine, sdf, slope_interval
 
 
 def implicit_from_params(params, mode, **kwargs):
     precision = kwargs.get('precision', 'fp32')
     recheck_tol = kwargs.get('recheck_tol', 0.0)
     if mode == 'sdf':
-        raise RuntimeError("mode 'sdf' (Lipschitz test, src/sdf.py) is not part of the range-analysis path")
+        return sdf.WeakSDFImplicitFunction(None, lipschitz_bound=kwargs.get('sdf_lipschitz', 1.)), params
     elif mode == 'interval':
         ctx = affine.AffineContext('interval')
     elif mode == 'affine_fixed':

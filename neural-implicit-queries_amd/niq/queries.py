@@ -56,9 +56,8 @@ def cast_rays(funcs_tuple, params_tuple, roots, dirs, opts, stream=None):
         params_tuple = (params_tuple,)
     if len(funcs_tuple) != len(params_tuple):
         raise ValueError("funcs_tuple and params_tuple lengths differ")
-    modes = {f.cfg.mode for f in funcs_tuple}
-    if len(modes) != 1:
-        raise ValueError("all functions of one cast_rays call must use the same range-analysis mode")
+    if len({bytes(f.cfg) for f in funcs_tuple}) != 1:
+        raise ValueError("all functions of one cast_rays call must use the same range-analysis mode and options")
     _lib.require_gpu()
     lib = _lib.load_library()
     roots = as_device_f32(roots)
@@ -92,8 +91,8 @@ def cast_rays_frustum(funcs_tuple, params_tuple, cam_params, opts, stream=None):
         params_tuple = (params_tuple,)
     if len(funcs_tuple) != len(params_tuple):
         raise ValueError("funcs_tuple and params_tuple lengths differ")
-    if len({f.cfg.mode for f in funcs_tuple}) != 1:
-        raise ValueError("all functions of one cast must use the same range-analysis mode")
+    if len({bytes(f.cfg) for f in funcs_tuple}) != 1:
+        raise ValueError("all functions of one cast must use the same range-analysis mode and options")
     _lib.require_gpu()
     lib = _lib.load_library()
     root_pos, look_dir, up_dir, left_dir, fov_x, fov_y, res_x, res_y = cam_params

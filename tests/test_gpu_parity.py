@@ -133,7 +133,7 @@ def test_general_boxes_match_oracle():
     assert np.all(rel_close(bounds.cpu().numpy()[:, 1], r_hi))
 
 
-@pytest.mark.parametrize("mode", ["affine_fixed", "interval", "slope_interval"])
+@pytest.mark.parametrize("mode", ["affine_fixed", "interval", "slope_interval", "sdf"])
 @pytest.mark.parametrize("name", ["c1_relu_3x32", "fox"])
 def test_tree_matches_oracle(mode, name):
     note(f"[tree {name} depth 12] {mode}")
@@ -355,6 +355,80 @@ def test_find_any_intersection_touching_half_spaces():
     fc, _ = implicit_mlp_utils.implicit_from_params(pc, "affine_fixed")
     found2, _, _, loc2 = kd_tree.find_any_intersection((fa, fc), (pa, pc), LO, UP, 0.05)
     assert not found2 and np.all(loc2.numpy() == -777.0)
+
+
+# ------------------------------------------------------------ weak SDF
+def spectral_lipschitz(p):
+    """Product of the dense layers' spectral norms (ReLU/tanh are 1-Lipschitz)."""
+    lip = 1.0
+    for k, v in p.items():
+        if k.endswith(".dense.A"):
+            lip *= float(np.linalg.norm(np.asarray(v, np.float64), 2))
+    return lip
+
+
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "small_tanh", "fox"])
+def test_sdf_classify_matches_oracle(name):
+    """The weak-SDF test (src/sdf.py:31-50) on axis and general boxes: codes
+    identical to the oracle's except where |f(centre)| lies within 1e-5 * scale
+    of lipschitz * radius or of the offset; bounds within 1e-5 * scale."""
+    note(f"[sdf classify] {name}")
+    p = model(name)
+    # the sample model was fit as a distance field: the default bound 1 applies
+    lip = 1.0 if name == "fox" else spectral_lipschitz(p)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "sdf", sdf_lipschitz=lip)
+    of = ofuncs.OracleFunc(p, "sdf", sdf_lipschitz=lip)
+    lo, up = boxes(6000, 17, 0.05)
+    codes, bounds = func.classify_box(p, torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda"),
+                                      offset=0.01, return_bounds=True)
+    codes, bounds = codes.cpu().numpy(), bounds.cpu().numpy()
+    ref = of.classify_box(lo, up, offset=0.01)
+    center, vecs = oaff.box_to_general(lo, up)
+    val = of(center).reshape(-1)
+    r = ofuncs.sdf_radius(vecs) * np.float32(lip)
+    scale = float(np.max(np.abs(val)))
+    near = (np.abs(np.abs(val) - r) <= 1e-5 * scale) | (np.abs(np.abs(val) - 0.01) <= 1e-5 * scale)
+    differ = codes != ref
+    note(f"  lipschitz {lip:.2f}; codes {np.bincount(codes, minlength=3).tolist()}; differ {int(differ.sum())}; "
+         f"near {int(near.sum())}")
+    assert np.all(near[differ])
+    assert (codes != 0).sum() > 0
+    assert np.all(rel_close(bounds[:, 0], val - r)) and np.all(rel_close(bounds[:, 1], val + r))
+    rng = np.random.default_rng(5)
+    c = rng.uniform(-1, 1, (3000, 3)).astype(np.float32)
+    v = (rng.normal(size=(3000, 3, 3)) * 0.01).astype(np.float32)
+    gc = func.classify_general_box(p, torch.as_tensor(c, device="cuda"), torch.as_tensor(v, device="cuda")).cpu().numpy()
+    gr = of.classify_general_box(c, v)
+    val = of(c).reshape(-1)
+    r = ofuncs.sdf_radius(v) * np.float32(lip)
+    near = np.abs(np.abs(val) - r) <= 1e-5 * scale
+    assert np.all(near[gc != gr])
+
+
+def test_sdf_rays_match_oracle():
+    """Ray casting with the weak-SDF step test: the same (t, hit_id, count) as
+    the oracle's for >= 99.5 % of rays."""
+    note("[sdf rays fox 64x64]")
+    p = model("fox")
+    lip = 1.0
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "sdf", sdf_lipschitz=lip)
+    roots, dirs = camera(64)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    t, hid, cnt, ne = queries.cast_rays((func,), (p,), roots, dirs, opts)
+    rt, rh, rc, rne = orays.cast_rays((ofuncs.OracleFunc(p, "sdf", sdf_lipschitz=lip),), roots, dirs, opts)
+    t, hid, cnt = t.cpu().numpy(), hid.cpu().numpy(), cnt.cpu().numpy()
+    same = (t == rt) & (hid == rh) & (cnt == rc)
+    note(f"  {same.mean() * 100:.3f}% rays identical; hits {int((hid > 0).sum())} / {int((rh > 0).sum())}; "
+         f"n_evals gpu {ne} oracle {rne}")
+    assert same.mean() >= 0.995
+    f2, _ = implicit_mlp_utils.implicit_from_params(p, "sdf", sdf_lipschitz=lip)
+    frt, frh, frc, fne = queries.cast_rays_frustum((f2,), (p,), frust_cam(64), dict(opts, n_side_init=8))
+    ort, orh, orc, one = ofr.cast_rays_frustum((ofuncs.OracleFunc(p, "sdf", sdf_lipschitz=lip),), *frust_cam(64),
+                                               dict(opts, n_side_init=8))
+    fsame = (frt.cpu().numpy() == ort) & (frh.cpu().numpy() == orh) & (frc.cpu().numpy() == orc)
+    note(f"  frustum: {fsame.mean() * 100:.2f}% pixels identical; N_evals {fne} / {one}")
+    assert fsame.mean() >= 0.99
 
 
 # ------------------------------------------------------------ sampling

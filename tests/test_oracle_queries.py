@@ -60,7 +60,7 @@ def rows(lo, up, valid):
     return {tuple(np.r_[a, b]) for a, b in zip(lo[valid], up[valid])}
 
 
-@pytest.mark.parametrize("mode", ["affine_fixed", "interval"])
+@pytest.mark.parametrize("mode", ["affine_fixed", "interval", "sdf"])
 def test_tree_matches_brute_force(mode):
     p = models.config_model("c1_relu_3x32")
     f = ofuncs.OracleFunc(p, mode)
