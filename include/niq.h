@@ -291,6 +291,45 @@ niq_status niq_bulk_sums_nodes(const niq_model* m, const float* unknown_lower, c
                                int64_t n_interior, int64_t n_samples, uint64_t seed, niq_bulk_sums* out,
                                void* stream);
 
+/* ---- normals and shading -------------------------------------------------------
+ * The image half of render_image (src/render.py:94-150). */
+
+/* hit_pos = roots + t * dirs (src/render.py:139), float32, unfused; (n, 3). */
+niq_status niq_hit_positions(const float* roots, const float* dirs, const float* t, int64_t n,
+                             float* out, void* stream);
+
+/* Replaces outward_normals with method='finite_differences' (src/render.py:53-90):
+ * per pixel the normalised gradient of function hit_id (1..n_models) from the
+ * 4 tetrahedron samples hit_pos + (+-eps, +-eps, +-eps); (0, 0, 0) where
+ * hit_id matches no function.  hit_pos, out_normals (n, 3) float32; hit_ids
+ * (n,) int32.  Asynchronous on `stream`. */
+niq_status niq_outward_normals(const niq_model* const* models, int n_models, const float* hit_pos,
+                               const int32_t* hit_ids, int64_t n, float eps, float* out_normals,
+                               void* stream);
+
+#define NIQ_SHADE_NORMAL 0
+#define NIQ_SHADE_MATCAP 1
+
+/* Replaces shade_image (src/render.py:160-224) and, with composite / tonemap
+ * set, the compositing and tonemap_image of render_image (:139-149, :152-158). */
+typedef struct niq_shade_opts {
+  int shading;                 /* NIQ_SHADE_NORMAL ('normal') or NIQ_SHADE_MATCAP ('matcap_color') */
+  float up_dir[3];
+  const float* matcaps[4];     /* device (h, w, 3) float32 images: r, g, b, k */
+  int matcap_h, matcap_w;
+  float colors[8][3];          /* shading_color_tuple, by hit_id */
+  int n_colors;
+  const float* color_per_hit;  /* device (n, 3): shading_color_func(hit_pos), or NULL */
+  int composite;               /* 1: white where hit_id == 0 */
+  int tonemap;                 /* 1: tonemap_image after compositing */
+  float gamma, white_level, exposure;
+} niq_shade_opts;
+
+/* ray_dirs, normals, out_rgb (n, 3) float32; hit_ids (n,) int32.
+ * Asynchronous on `stream`. */
+niq_status niq_shade_image(const niq_shade_opts* opts, const float* ray_dirs, const float* normals,
+                           const int32_t* hit_ids, int64_t n, float* out_rgb, void* stream);
+
 /* ---- serving loop ------------------------------------------------------------
  * No counterpart in the reference (it runs one query at a time from one Python
  * thread, src/queries.py:134, src/kd_tree.py:103,766).  A server owns

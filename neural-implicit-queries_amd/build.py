@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "niq", "libniq.so")
 SOURCES = ["api_model.hip", "api_classify.hip", "api_classify_bf16.hip", "api_classify_rows.hip", "api_tree.hip", "api_rays.hip", "api_cp_mc.hip",
            "api_serve.hip", "api_frustum.hip",
-           "api_isect.hip", "api_sample.hip"]
+           "api_isect.hip", "api_sample.hip", "api_render.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -28,7 +28,8 @@ EXPORTS = ("niq_model_create", "niq_model_destroy", "niq_model_device_bytes",
            "niq_tree_workspace_bytes", "niq_tree_build", "niq_tree_enqueue",
            "niq_tree_counters_bytes", "niq_tree_decode", "niq_cast_workspace_bytes",
            "niq_cast_rays", "niq_cast_rays_frustum", "niq_find_any_intersection", "niq_closest_point",
-           "niq_sample_workspace_bytes", "niq_sample_surface_nodes", "niq_bulk_sums_nodes", "niq_mc_extract", "niq_mc_table",
+           "niq_sample_workspace_bytes", "niq_sample_surface_nodes", "niq_bulk_sums_nodes",
+           "niq_hit_positions", "niq_outward_normals", "niq_shade_image", "niq_mc_extract", "niq_mc_table",
            "niq_serve_create", "niq_serve_destroy", "niq_serve_submit_rays", "niq_serve_submit_tree",
            "niq_serve_submit_closest_point", "niq_serve_poll", "niq_serve_bytes_in_use",
            "niq_last_error", "niq_version")
@@ -96,6 +97,14 @@ class ServeInfo(ctypes.Structure):
                 ("graph_replay", ctypes.c_int32), ("n_processed", ctypes.c_int64)]
 
 
+class ShadeOpts(ctypes.Structure):
+    _fields_ = [("shading", ctypes.c_int), ("up_dir", ctypes.c_float * 3),
+                ("matcaps", ctypes.c_void_p * 4), ("matcap_h", ctypes.c_int), ("matcap_w", ctypes.c_int),
+                ("colors", (ctypes.c_float * 3) * 8), ("n_colors", ctypes.c_int),
+                ("color_per_hit", ctypes.c_void_p), ("composite", ctypes.c_int), ("tonemap", ctypes.c_int),
+                ("gamma", ctypes.c_float), ("white_level", ctypes.c_float), ("exposure", ctypes.c_float)]
+
+
 class BulkSums(ctypes.Structure):
     _fields_ = [("interior_volume", ctypes.c_double), ("interior_moment", ctypes.c_double * 3),
                 ("unknown_volume", ctypes.c_double), ("n_inside", ctypes.c_double),
@@ -138,6 +147,9 @@ def load_library():
         "niq_sample_surface_nodes": (I, [P, P, P, I64, F, ctypes.c_uint64, I64, I64, I64, P, I64,
                                          ctypes.POINTER(I64), ctypes.POINTER(I64), P, S, P]),
         "niq_bulk_sums_nodes": (I, [P, P, P, I64, P, P, I64, I64, ctypes.c_uint64, P, P]),
+        "niq_hit_positions": (I, [P, P, P, I64, P, P]),
+        "niq_outward_normals": (I, [P, I, P, P, I64, F, P, P]),
+        "niq_shade_image": (I, [P, P, P, P, I64, P, P]),
         "niq_serve_create": (I, [I, S, I64, ctypes.POINTER(P)]),
         "niq_serve_destroy": (I, [P]),
         "niq_serve_submit_rays": (I, [P, P, I, P, P, P, I64, P, P, P, P, ctypes.POINTER(ctypes.c_uint64)]),

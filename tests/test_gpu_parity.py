@@ -25,8 +25,9 @@ import pytest
 import torch
 
 from oracle import funcs as ofuncs, tree as otree, rays as orays, mlp as omlp, affine as oaff
-from oracle import frustum as ofr, intersect as oisect, sampling as osamp
+from oracle import frustum as ofr, intersect as oisect, sampling as osamp, render as orender
 from niq import models, mlp as nmlp, implicit_mlp_utils, kd_tree, queries, render
+from niq import _lib as _lib_mod
 
 pytestmark = pytest.mark.gpu
 
@@ -429,6 +430,107 @@ def test_sdf_rays_match_oracle():
     fsame = (frt.cpu().numpy() == ort) & (frh.cpu().numpy() == orh) & (frc.cpu().numpy() == orc)
     note(f"  frustum: {fsame.mean() * 100:.2f}% pixels identical; N_evals {fne} / {one}")
     assert fsame.mean() >= 0.99
+
+
+# ------------------------------------------------------------ normals and shading
+def test_normals_match_oracle():
+    """outward_normals (src/render.py:53-90) at the GPU's own fox hits: within
+    2e-3 of the oracle's (the tetrahedron difference divides value rounding of
+    ~1e-7 by eps = 1e-3), zeros where nothing was hit; a plane's normal exactly."""
+    note("[normals fox]")
+    p = model("fox")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    roots, dirs = camera(96)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    t, hid, _, _ = queries.cast_rays((func,), (p,), roots, dirs, opts)
+    lib = _lib_mod.load_library()
+    rt_, dt_ = torch.as_tensor(roots, device="cuda"), torch.as_tensor(dirs, device="cuda")
+    pos = torch.empty_like(rt_)
+    _lib_mod.check(lib.niq_hit_positions(_lib_mod.ptr(rt_), _lib_mod.ptr(dt_), _lib_mod.ptr(t), rt_.shape[0],
+                                         _lib_mod.ptr(pos), _lib_mod.stream_handle()))
+    want_pos = roots + t.cpu().numpy()[:, None] * dirs
+    assert np.array_equal(pos.cpu().numpy(), want_pos)
+    nr = render.outward_normals((func,), (p,), pos, hid, 1e-3).cpu().numpy()
+    ref = orender.outward_normals((ofuncs.OracleFunc(p),), pos.cpu().numpy(), hid.cpu().numpy(), 1e-3)
+    h = hid.cpu().numpy() > 0
+    err = np.abs(nr - ref).max(axis=1)
+    note(f"  {int(h.sum())} hits; normal error max {err[h].max():.2e} mean {err[h].mean():.2e}")
+    assert np.all(nr[~h] == 0) and np.all(err[h] <= 2e-3)
+    nv = np.array([0.36, -0.48, 0.8], np.float32)
+    A1 = np.concatenate([np.eye(3), -np.eye(3)], axis=1)
+    A2 = np.concatenate([0.5 * nv, -0.5 * nv])[:, None]
+    pl = nmlp.build_spec([nmlp.dense(3, 6, A=A1, b=np.zeros(6)), nmlp.dense(6, 1, A=A2, b=[-0.1]), nmlp.squeeze_last()])
+    fpl, _ = implicit_mlp_utils.implicit_from_params(pl, "affine_fixed")
+    q = torch.rand((500, 3), device="cuda") * 2 - 1
+    n2 = render.outward_normals((func, fpl), (p, pl), q, torch.full((500,), 2), 1e-3).cpu().numpy()
+    assert np.allclose(n2, nv, atol=1e-3)
+
+
+def test_shading_matches_oracle():
+    """shade_image (src/render.py:160-224) with normal and matcap shading, and
+    render_image's compositing and tonemap: within 1e-5 of the oracle."""
+    note("[shading]")
+    rng = np.random.default_rng(8)
+    n = 5000
+    dirs = orender.normalize(rng.normal(size=(n, 3)).astype(np.float32))
+    nr = orender.normalize(rng.normal(size=(n, 3)).astype(np.float32))
+    ids = rng.integers(0, 3, n).astype(np.int32)
+    mats = tuple(rng.uniform(0, 1, (40, 56, 3)).astype(np.float32) for _ in range(4))
+    up = np.array([0.0, 1.0, 0.0], np.float32)
+    cols = ((0.157, 0.613, 1.0), (0.9, 0.1, 0.2))
+    g = lambda a: torch.as_tensor(a, device="cuda")
+    got = render.shade_image("matcap_color", g(dirs), None, g(nr), g(ids), up, tuple(g(m) for m in mats), cols)
+    ref = orender.shade_image("matcap_color", dirs, None, nr, ids, up, mats, cols)
+    note(f"  matcap max error {np.abs(got.cpu().numpy() - ref).max():.2e}")
+    assert np.allclose(got.cpu().numpy(), ref, atol=1e-5)
+    got = render.shade_image("normal", None, None, g(nr), g(ids), up, None, cols)
+    assert np.allclose(got.cpu().numpy(), orender.shade_image("normal", None, None, nr, ids, up, None, cols), atol=1e-7)
+    img = render._shade("matcap_color", g(dirs), None, g(nr), g(ids), up, tuple(g(m) for m in mats), cols, None,
+                        composite=True, tonemap=True).cpu().numpy()
+    want = orender.composite(ref, ids, tonemap=True)
+    # colours below zero (c_k < 0 mixes) tonemap to NaN in both, as in the reference
+    assert np.allclose(img, want, atol=1e-5, equal_nan=True)
+
+
+@pytest.mark.parametrize("frustum", [False, True])
+def test_render_image_matches_oracle(frustum):
+    """render_image (src/render.py:94-150) of the fox with matcap shading: the
+    same hit ids and depths as the oracle's pipeline for >= 99 % of pixels,
+    colours within 2e-3 where both agree on the hit."""
+    note(f"[render_image fox frustum={frustum}]")
+    p = model("fox")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    eye = np.array([0.0, 0.3, 2.5], np.float32)
+    look, up, left = render.look_at(eye)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    opts["n_side_init"] = 8
+    res = 64
+    rng = np.random.default_rng(9)
+    mats = tuple(rng.uniform(0, 1, (32, 32, 3)).astype(np.float32) for _ in range(4))
+    img, depth, counts, hid, ne, _ = render.render_image((func,), (p,), eye, look, up, left, res, 45.0, frustum,
+                                                         opts, shading="matcap_color", matcaps=mats, tonemap=True)
+    roots, dirs = render.generate_camera_rays(eye, look, up, res=res, fov_deg=45.0, device=None)
+    of = ofuncs.OracleFunc(p)
+    if frustum:
+        rt, rh, _, _ = ofr.cast_rays_frustum((of,), eye, look, up, left, 45.0, 45.0, res, res, opts)
+        rt, rh = rt.T.reshape(-1), rh.T.reshape(-1)
+    else:
+        rt, rh, _, _ = orays.cast_rays((of,), roots, dirs, opts)
+    pos = (roots + rt[:, None] * dirs).astype(np.float32)
+    nrm = orender.outward_normals((of,), pos, rh, opts["hit_eps"])
+    col = orender.shade_image("matcap_color", dirs, pos, nrm, rh, up, mats, ((0.157, 0.613, 1.0),))
+    want = orender.composite(col, rh, tonemap=True).reshape(res, res, 3)
+    hid, depth, img = hid.cpu().numpy().reshape(-1), depth.cpu().numpy().reshape(-1), img.cpu().numpy()
+    same = (hid == rh) & (depth == rt)
+    # negative mixes (c_k < 0) tonemap to NaN in both pipelines, as in the reference
+    ok = np.isclose(img.reshape(-1, 3), want.reshape(-1, 3), rtol=0, atol=2e-3, equal_nan=True).all(axis=1)
+    err = np.nan_to_num(np.abs(img.reshape(-1, 3) - want.reshape(-1, 3))).max(axis=1)
+    note(f"  {same.mean() * 100:.2f}% pixels with the same hit and depth; {int((hid > 0).sum())} hits; "
+         f"colour error there max {err[same].max():.2e}")
+    assert same.mean() >= 0.99 and (hid > 0).sum() > 50
+    assert np.all(ok[same])
 
 
 # ------------------------------------------------------------ sampling
