@@ -14,6 +14,7 @@
 // XOR-swizzled by the box so that both the 16-byte operand reads and the
 // 16-byte result writes are bank-conflict free.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
@@ -362,7 +363,8 @@ __device__ __forceinline__ void mlp_forward(const DevModel& M, float* __restrict
   constexpr int BT = NB / 16;
   constexpr int FTMAX = (WMAX / 16 + NW - 1) / NW;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform, so the feature-tile tests below are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ln = lane & 15;
   const int lg = lane >> 4;
 
@@ -381,43 +383,68 @@ __device__ __forceinline__ void mlp_forward(const DevModel& M, float* __restrict
 
     __syncthreads();  // layer input complete in X
 
-    floatx4 a_nx[FTMAX];
+    // Contraction over k-blocks of 16 inputs.  Each accumulator sums its
+    // k-blocks in ascending order; the next k-block's weights (global) and
+    // input fragments (LDS, unchanged until the barrier below) are loaded
+    // while the current one's MFMAs run.  `FULL`: every feature tile of this
+    // wave exists, so the loop carries no per-tile test.
+    auto contract = [&](auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
+      floatx4 a_nx[FTMAX];
+      floatx4 b_nx[R][BT];
 #pragma unroll
-    for (int f = 0; f < FTMAX; ++f) {
-      const int ft = wave + f * NW;
-      if (ft < nft) a_nx[f] = Ly.Wp[(size_t)(ft * nkb) * 64 + lane];
-    }
-    for (int kb = 0; kb < nkb; ++kb) {
-      floatx4 a[FTMAX];
-#pragma unroll
-      for (int f = 0; f < FTMAX; ++f) a[f] = a_nx[f];
-      if (kb + 1 < nkb) {
-#pragma unroll
-        for (int f = 0; f < FTMAX; ++f) {
-          const int ft = wave + f * NW;
-          if (ft < nft) a_nx[f] = Ly.Wp[(size_t)(ft * nkb + kb + 1) * 64 + lane];
-        }
+      for (int f = 0; f < FTMAX; ++f) {
+        const int ft = wave + f * NW;
+        if (FULL || ft < nft) a_nx[f] = Ly.Wp[(size_t)(ft * nkb) * 64 + lane];
       }
 #pragma unroll
-      for (int p = 0; p < R; ++p) {
+      for (int p = 0; p < R; ++p)
 #pragma unroll
-        for (int t = 0; t < BT; ++t) {
-          const int b = t * 16 + ln;
-          const floatx4 bx = *reinterpret_cast<const floatx4*>(&X[xoff<NB, WMAX>(p, b, kb * 16 + lg * 4)]);
+        for (int t = 0; t < BT; ++t)
+          b_nx[p][t] = *reinterpret_cast<const floatx4*>(&X[xoff<NB, WMAX>(p, t * 16 + ln, lg * 4)]);
+      for (int kb = 0; kb < nkb; ++kb) {
+        floatx4 a[FTMAX];
+        floatx4 bx[R][BT];
+#pragma unroll
+        for (int f = 0; f < FTMAX; ++f) a[f] = a_nx[f];
+#pragma unroll
+        for (int p = 0; p < R; ++p)
+#pragma unroll
+          for (int t = 0; t < BT; ++t) bx[p][t] = b_nx[p][t];
+        if (kb + 1 < nkb) {
 #pragma unroll
           for (int f = 0; f < FTMAX; ++f) {
             const int ft = wave + f * NW;
-            if (ft < nft) {
+            if (FULL || ft < nft) a_nx[f] = Ly.Wp[(size_t)(ft * nkb + kb + 1) * 64 + lane];
+          }
 #pragma unroll
-              for (int j = 0; j < 4; ++j) {
-                const float av = PL::absw(p) ? fabsf(a[f][j]) : a[f][j];
-                acc[f][p][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bx[j], acc[f][p][t], 0, 0, 0);
+          for (int p = 0; p < R; ++p)
+#pragma unroll
+            for (int t = 0; t < BT; ++t)
+              b_nx[p][t] = *reinterpret_cast<const floatx4*>(
+                  &X[xoff<NB, WMAX>(p, t * 16 + ln, (kb + 1) * 16 + lg * 4)]);
+        }
+#pragma unroll
+        for (int p = 0; p < R; ++p) {
+#pragma unroll
+          for (int t = 0; t < BT; ++t) {
+#pragma unroll
+            for (int f = 0; f < FTMAX; ++f) {
+              const int ft = wave + f * NW;
+              if (FULL || ft < nft) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                  const float av = PL::absw(p) ? fabsf(a[f][j]) : a[f][j];
+                  acc[f][p][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bx[p][t][j], acc[f][p][t], 0, 0, 0);
+                }
               }
             }
           }
         }
       }
-    }
+    };
+    if (wave + (FTMAX - 1) * NW < nft) contract(std::true_type{});
+    else contract(std::false_type{});
 
     __syncthreads();  // every wave has consumed X
 

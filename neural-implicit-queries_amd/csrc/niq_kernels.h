@@ -26,6 +26,17 @@ constexpr int kLdsLimit = 160 * 1024;
 template <int R, int NB, int WMAX>
 struct Fits { static constexpr bool value = (size_t)R * NB * WMAX * 4 <= (size_t)kLdsLimit; };
 
+// Waves per SIMD that the LDS of a tile admits (workgroups per CU x waves per
+// workgroup / 4 SIMDs), capped at 4 (128 VGPRs a wave): the register budget
+// given to the compiler, so registers do not cut occupancy below what LDS
+// allows.
+template <int R, int NB, int NW, int WMAX>
+struct LdsOcc {
+  static constexpr int wgs = (int)((size_t)kLdsLimit / ((size_t)R * NB * WMAX * 4));
+  static constexpr int raw = wgs * NW / 4;
+  static constexpr int value = raw < 1 ? 1 : (raw > 4 ? 4 : raw);
+};
+
 // ----------------------------------------------------------- classification
 struct ClassifyArgs {
   const float* lower;      // axis-aligned boxes (n, d) or null
@@ -94,7 +105,7 @@ __device__ __forceinline__ void fill_box_inputs(float* X, const ClassifyArgs& a,
 }
 
 template <int KIND, int V, int NB, int NW, int WMAX>
-__global__ void __launch_bounds__(NW * 64)
+__global__ void __launch_bounds__(NW * 64, (LdsOcc<Planes<KIND, V, 0>::R, NB, NW, WMAX>::value))
 k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
   using PL = Planes<KIND, V, 0>;
   constexpr int NT = NW * 64;
@@ -508,7 +519,7 @@ struct RayArgs {
 // weak-SDF form: three point planes (segment centre, start, start + hit_eps)
 // and the segment radius |half_vec| kept in LDS per slot.
 template <int KIND, int V, int NB, int NW, int WMAX>
-__global__ void __launch_bounds__(NW * 64)
+__global__ void __launch_bounds__(NW * 64, (LdsOcc<Planes<KIND, V, KIND == 2 ? 3 : 2>::R, NB, NW, WMAX>::value))
 k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
   using PL = Planes<KIND, V, KIND == 2 ? 3 : 2>;
   constexpr int R = PL::R;
