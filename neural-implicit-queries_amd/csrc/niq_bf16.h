@@ -281,7 +281,8 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
   constexpr int BT = NB / 16;
   constexpr int FTMAX = (WMAX / 16 + NW - 1) / NW;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform, so the feature-tile tests below are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ln = lane & 15;
   const int lg = lane >> 4;
   const floatx4 z4 = {0.f, 0.f, 0.f, 0.f};
@@ -300,96 +301,101 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
         for (int t = 0; t < BT; ++t) acc[f][p][t] = z4;
 
     __syncthreads();
-    // FP32 weight fragments are fetched one K chunk ahead
-    floatx4 wpf[FTMAX][2];
-    if constexpr (WS == WSRC_FP32) {
-#pragma unroll
-      for (int f = 0; f < FTMAX; ++f) {
+    // Weight fragments are fetched one K chunk ahead: FP32 fragments (split
+    // here) or the packed bf16 copies.  `FULL`: every feature tile of this
+    // wave exists, so the loop carries no per-tile test.
+    auto contract = [&](auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
+      floatx4 wpf[FTMAX][2];
+      bf16x8 wnx[FTMAX][4];
+      auto fetch = [&](int f, int kb) {
         const int ft = wave + f * NW;
-        if (ft < nft) {
-          const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb) * 64 + lane;
+        if constexpr (WS == WSRC_FP32) {
+          const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb + kb) * 64 + lane;
           wpf[f][0] = P[0];
           wpf[f][1] = P[64];
-        }
-      }
-    }
-    for (int kb = 0; kb < nkb; ++kb) {
-#pragma unroll
-      for (int t = 0; t < BT; ++t) {
-        const int b = t * 16 + ln;
-        const int k0 = kb * 32 + lg * 8;
-        const bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(0, b, k0)]);
-        const bf16x8 x2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(1, b, k0)]);
-        const bf16x8 x3 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(2, b, k0)]);
-        const bf16x8 xe = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(PB::E, b, k0)]);
-#pragma unroll
-        for (int f = 0; f < FTMAX; ++f) {
-          const int ft = wave + f * NW;
-          if (ft >= nft) continue;
+        } else {
           const size_t o = (size_t)(ft * nkb + kb) * 64 + lane;
-          bf16x8 w1, w2, w3, wa1, wa2;
-          if constexpr (WS == WSRC_FP32) {
-            // split this chunk's fragment, then fetch the next chunk's into
-            // the same registers
-            split_w8(wpf[f][0], wpf[f][1], w1, w2, w3, wa1, wa2);
-            if (t + 1 == BT && kb + 1 < nkb) {
-              const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb + kb + 1) * 64 + lane;
-              wpf[f][0] = P[0];
-              wpf[f][1] = P[64];
-            }
-          } else {
-            w1 = Ly.W1[o]; w2 = Ly.W2[o]; w3 = Ly.W3[o];
-          }
-          // base row, smallest products first, fresh accumulator per chunk
-          {
-            floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x3, z4, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, x2, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, x1, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x2, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, x1, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x1, c, 0, 0, 0);
-            acc[f][0][t] += c;
-          }
+          wnx[f][0] = Ly.W1[o]; wnx[f][1] = Ly.W2[o]; wnx[f][2] = Ly.W3[o]; wnx[f][3] = Ly.Wa[o];
+        }
+      };
 #pragma unroll
-          for (int i = 0; i < V; ++i) {
-            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k0)]);
-            const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k0)]);
-            floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a2, z4, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, a1, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a1, c, 0, 0, 0);
-            acc[f][1 + i][t] += c;
-          }
-          if constexpr (WS == WSRC_FP32) {
-            // non-negative terms: one accumulator over the whole K
-            acc[f][NS][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xe, acc[f][NS][t], 0, 0, 0);
-            acc[f][NS][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xe, acc[f][NS][t], 0, 0, 0);
-          } else {
-            acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ly.Wa[o], xe, z4, 0, 0, 0);
-          }
-          // plain point rows (hit test): the same split products, no bound
+      for (int f = 0; f < FTMAX; ++f)
+        if (FULL || wave + f * NW < nft) fetch(f, 0);
+      for (int kb = 0; kb < nkb; ++kb) {
 #pragma unroll
-          for (int r = 0; r < NP; ++r) {
-            const int pl = PB::NPL + r * NPS;
-            const bf16x8 q1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl, b, k0)]);
-            const bf16x8 q2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl + 1, b, k0)]);
-            floatx4 c;
-            if constexpr (NPS == 3) {
-              const bf16x8 q3 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl + 2, b, k0)]);
-              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q3, z4, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q2, c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, q1, c, 0, 0, 0);
+        for (int t = 0; t < BT; ++t) {
+          const int b = t * 16 + ln;
+          const int k0 = kb * 32 + lg * 8;
+          const bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(0, b, k0)]);
+          const bf16x8 x2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(1, b, k0)]);
+          const bf16x8 x3 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(2, b, k0)]);
+          const bf16x8 xe = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(PB::E, b, k0)]);
+#pragma unroll
+          for (int f = 0; f < FTMAX; ++f) {
+            const int ft = wave + f * NW;
+            if (!FULL && ft >= nft) continue;
+            bf16x8 w1, w2, w3, wa1, wa2;
+            if constexpr (WS == WSRC_FP32) {
+              split_w8(wpf[f][0], wpf[f][1], w1, w2, w3, wa1, wa2);
             } else {
-              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, q1, z4, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q2, c, 0, 0, 0);
+              w1 = wnx[f][0]; w2 = wnx[f][1]; w3 = wnx[f][2]; wa1 = wnx[f][3];
             }
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q2, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q1, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q1, c, 0, 0, 0);
-            acc[f][R + r][t] += c;
+            // the next chunk's fragment goes into the same registers
+            if (t + 1 == BT && kb + 1 < nkb) fetch(f, kb + 1);
+            // base row, smallest products first, fresh accumulator per chunk
+            {
+              floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x3, z4, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, x2, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, x1, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x2, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, x1, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x1, c, 0, 0, 0);
+              acc[f][0][t] += c;
+            }
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+              const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k0)]);
+              const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k0)]);
+              floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a2, z4, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, a1, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a1, c, 0, 0, 0);
+              acc[f][1 + i][t] += c;
+            }
+            if constexpr (WS == WSRC_FP32) {
+              // non-negative terms: one accumulator over the whole K
+              acc[f][NS][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2, xe, acc[f][NS][t], 0, 0, 0);
+              acc[f][NS][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xe, acc[f][NS][t], 0, 0, 0);
+            } else {
+              acc[f][NS][t] += __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, xe, z4, 0, 0, 0);
+            }
+            // plain point rows (hit test): the same split products, no bound
+#pragma unroll
+            for (int r = 0; r < NP; ++r) {
+              const int pl = PB::NPL + r * NPS;
+              const bf16x8 q1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl, b, k0)]);
+              const bf16x8 q2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl + 1, b, k0)]);
+              floatx4 c;
+              if constexpr (NPS == 3) {
+                const bf16x8 q3 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(pl + 2, b, k0)]);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q3, z4, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q2, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, q1, c, 0, 0, 0);
+              } else {
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3, q1, z4, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q2, c, 0, 0, 0);
+              }
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q2, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, q1, c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, q1, c, 0, 0, 0);
+              acc[f][R + r][t] += c;
+            }
           }
         }
       }
-    }
+    };
+    if (wave + (FTMAX - 1) * NW < nft) contract(std::true_type{});
+    else contract(std::false_type{});
 
     __syncthreads();
 
