@@ -391,17 +391,11 @@ __device__ __forceinline__ void mlp_forward(const DevModel& M, float* __restrict
     auto contract = [&](auto full_tag) {
       constexpr bool FULL = decltype(full_tag)::value;
       floatx4 a_nx[FTMAX];
-      floatx4 b_nx[R][BT];
 #pragma unroll
       for (int f = 0; f < FTMAX; ++f) {
         const int ft = wave + f * NW;
         if (FULL || ft < nft) a_nx[f] = Ly.Wp[(size_t)(ft * nkb) * 64 + lane];
       }
-#pragma unroll
-      for (int p = 0; p < R; ++p)
-#pragma unroll
-        for (int t = 0; t < BT; ++t)
-          b_nx[p][t] = *reinterpret_cast<const floatx4*>(&X[xoff<NB, WMAX>(p, t * 16 + ln, lg * 4)]);
       for (int kb = 0; kb < nkb; ++kb) {
         floatx4 a[FTMAX];
         floatx4 bx[R][BT];
@@ -410,19 +404,14 @@ __device__ __forceinline__ void mlp_forward(const DevModel& M, float* __restrict
 #pragma unroll
         for (int p = 0; p < R; ++p)
 #pragma unroll
-          for (int t = 0; t < BT; ++t) bx[p][t] = b_nx[p][t];
+          for (int t = 0; t < BT; ++t)
+            bx[p][t] = *reinterpret_cast<const floatx4*>(&X[xoff<NB, WMAX>(p, t * 16 + ln, kb * 16 + lg * 4)]);
         if (kb + 1 < nkb) {
 #pragma unroll
           for (int f = 0; f < FTMAX; ++f) {
             const int ft = wave + f * NW;
             if (FULL || ft < nft) a_nx[f] = Ly.Wp[(size_t)(ft * nkb + kb + 1) * 64 + lane];
           }
-#pragma unroll
-          for (int p = 0; p < R; ++p)
-#pragma unroll
-            for (int t = 0; t < BT; ++t)
-              b_nx[p][t] = *reinterpret_cast<const floatx4*>(
-                  &X[xoff<NB, WMAX>(p, t * 16 + ln, (kb + 1) * 16 + lg * 4)]);
         }
 #pragma unroll
         for (int p = 0; p < R; ++p) {
