@@ -425,73 +425,74 @@ __device__ __forceinline__ void store3(float* p, long long i, float x, float y, 
 }
 
 // Stable per-batch placement of children and finished leaves, in the
-// reference's order (src/kd_tree.py:46-96, src/bucketing.py:16-32).
+// reference's order (src/kd_tree.py:46-96, src/bucketing.py:16-32).  The batch
+// is walked in chunks of 256 consecutive nodes, one per thread (coalesced
+// reads and writes); ranks within a chunk come from wave ballots.
 static __global__ void __launch_bounds__(256)
 k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __restrict__ boff,
                const TreeCounters* __restrict__ C, int level, int B, int split) {
   const int n = C->n[level & 1];
   const long long start = (long long)blockIdx.x * B;
   if (start >= n) return;
-  const long long end = min(start + (long long)B, (long long)n);
-  const int cnt = (int)(end - start);
-  const int per = (cnt + 255) / 256;
-  const int my0 = threadIdx.x * per;
-  const int my1 = min(my0 + per, cnt);
-  int lu = 0, lp = 0, ln = 0;
-  for (int j = my0; j < my1; ++j) {
-    const int c = a.codes[start + j];
-    lu += (c == CODE_UNKNOWN); lp += (c == CODE_POSITIVE); ln += (c == CODE_NEGATIVE);
-  }
-  __shared__ int s[3][256];
-  s[0][threadIdx.x] = lu; s[1][threadIdx.x] = lp; s[2][threadIdx.x] = ln;
-  __syncthreads();
-  for (int w = 1; w < 256; w <<= 1) {
-    int a0 = 0, a1 = 0, a2 = 0;
-    if ((int)threadIdx.x >= w) { a0 = s[0][threadIdx.x - w]; a1 = s[1][threadIdx.x - w]; a2 = s[2][threadIdx.x - w]; }
-    __syncthreads();
-    s[0][threadIdx.x] += a0; s[1][threadIdx.x] += a1; s[2][threadIdx.x] += a2;
-    __syncthreads();
-  }
-  int ru = s[0][threadIdx.x] - lu, rp = s[1][threadIdx.x] - lp, rn = s[2][threadIdx.x] - ln;
+  const int cnt = (int)min((long long)B, (long long)n - start);
   const long long Ub = boff[blockIdx.x * 4 + 0];
   const long long nUb = bcnt[blockIdx.x * 4 + 0];
   const long long Eb = boff[blockIdx.x * 4 + 1];
   const long long Ib = boff[blockIdx.x * 4 + 2];
-  for (int j = my0; j < my1; ++j) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  __shared__ int wsum[3][4];
+  int ru0 = 0, rp0 = 0, rn0 = 0;   // nodes of each kind before this chunk
+  for (int c0 = 0; c0 < cnt; c0 += 256) {
+    const int j = c0 + (int)threadIdx.x;
+    const bool in = j < cnt;
     const long long i = start + j;
-    const int c = a.codes[i];
-    const float l0 = a.lower[3 * i], l1 = a.lower[3 * i + 1], l2 = a.lower[3 * i + 2];
-    const float u0 = a.upper[3 * i], u1 = a.upper[3 * i + 1], u2 = a.upper[3 * i + 2];
-    if (c == CODE_UNKNOWN) {
-      if (split) {
-        const float e0 = u0 - l0, e1 = u1 - l1, e2 = u2 - l2;
-        int dim = 0; float em = e0;
-        if (e1 > em) { dim = 1; em = e1; }
-        if (e2 > em) { dim = 2; }
-        const float m0 = 0.5f * (l0 + u0), m1 = 0.5f * (l1 + u1), m2 = 0.5f * (l2 + u2);
-        const long long pa = 2 * Ub + ru, pb = 2 * Ub + nUb + ru;
-        if (pa < a.cap_next) {
-          store3(a.next_lower, pa, l0, l1, l2);
-          store3(a.next_upper, pa, dim == 0 ? m0 : u0, dim == 1 ? m1 : u1, dim == 2 ? m2 : u2);
-        }
-        if (pb < a.cap_next) {
-          store3(a.next_lower, pb, dim == 0 ? m0 : l0, dim == 1 ? m1 : l1, dim == 2 ? m2 : l2);
-          store3(a.next_upper, pb, u0, u1, u2);
-        }
-      } else {
-        const long long p = Ub + ru;
-        if (p < a.cap_next) { store3(a.next_lower, p, l0, l1, l2); store3(a.next_upper, p, u0, u1, u2); }
-      }
-      ++ru;
-    } else if (c == CODE_POSITIVE) {
-      const long long p = Eb + rp;
-      if (a.ext_lower && p < a.cap_ext) { store3(a.ext_lower, p, l0, l1, l2); store3(a.ext_upper, p, u0, u1, u2); }
-      ++rp;
-    } else {
-      const long long p = Ib + rn;
-      if (a.int_lower && p < a.cap_int) { store3(a.int_lower, p, l0, l1, l2); store3(a.int_upper, p, u0, u1, u2); }
-      ++rn;
+    const int c = in ? a.codes[i] : -1;
+    const unsigned long long mu = __ballot(c == CODE_UNKNOWN);
+    const unsigned long long mp = __ballot(c == CODE_POSITIVE);
+    const unsigned long long mn = __ballot(c == CODE_NEGATIVE);
+    if (lane == 0) { wsum[0][wave] = __popcll(mu); wsum[1][wave] = __popcll(mp); wsum[2][wave] = __popcll(mn); }
+    __syncthreads();
+    int pu = 0, pp = 0, pn = 0, tu = 0, tp = 0, tn = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      if (w < wave) { pu += wsum[0][w]; pp += wsum[1][w]; pn += wsum[2][w]; }
+      tu += wsum[0][w]; tp += wsum[1][w]; tn += wsum[2][w];
     }
+    __syncthreads();   // wsum is rewritten by the next chunk
+    if (in) {
+      const float l0 = a.lower[3 * i], l1 = a.lower[3 * i + 1], l2 = a.lower[3 * i + 2];
+      const float u0 = a.upper[3 * i], u1 = a.upper[3 * i + 1], u2 = a.upper[3 * i + 2];
+      if (c == CODE_UNKNOWN) {
+        const long long ru = ru0 + pu + __popcll(mu & below);
+        if (split) {
+          const float e0 = u0 - l0, e1 = u1 - l1, e2 = u2 - l2;
+          int dim = 0; float em = e0;
+          if (e1 > em) { dim = 1; em = e1; }
+          if (e2 > em) { dim = 2; }
+          const float m0 = 0.5f * (l0 + u0), m1 = 0.5f * (l1 + u1), m2 = 0.5f * (l2 + u2);
+          const long long pa = 2 * Ub + ru, pb = 2 * Ub + nUb + ru;
+          if (pa < a.cap_next) {
+            store3(a.next_lower, pa, l0, l1, l2);
+            store3(a.next_upper, pa, dim == 0 ? m0 : u0, dim == 1 ? m1 : u1, dim == 2 ? m2 : u2);
+          }
+          if (pb < a.cap_next) {
+            store3(a.next_lower, pb, dim == 0 ? m0 : l0, dim == 1 ? m1 : l1, dim == 2 ? m2 : l2);
+            store3(a.next_upper, pb, u0, u1, u2);
+          }
+        } else {
+          const long long p = Ub + ru;
+          if (p < a.cap_next) { store3(a.next_lower, p, l0, l1, l2); store3(a.next_upper, p, u0, u1, u2); }
+        }
+      } else if (c == CODE_POSITIVE) {
+        const long long p = Eb + rp0 + pp + __popcll(mp & below);
+        if (a.ext_lower && p < a.cap_ext) { store3(a.ext_lower, p, l0, l1, l2); store3(a.ext_upper, p, u0, u1, u2); }
+      } else if (c == CODE_NEGATIVE) {
+        const long long p = Ib + rn0 + pn + __popcll(mn & below);
+        if (a.int_lower && p < a.cap_int) { store3(a.int_lower, p, l0, l1, l2); store3(a.int_upper, p, u0, u1, u2); }
+      }
+    }
+    ru0 += tu; rp0 += tp; rn0 += tn;
   }
 }
 
