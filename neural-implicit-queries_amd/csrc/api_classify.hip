@@ -33,6 +33,17 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
 template <int KIND, int V>
 niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s,
                            int variant = 0) {
+  // at most two tiles per CU: latency-bound, so the wide tile (same sums in
+  // the same order, bit-identical results)
+  if (variant == 0 && max_n <= 32LL * num_cus()) {
+    switch (m->wt) {
+      case 64: return launch_classify_w<KIND, V, 64, TileCfgWide<64>>(m, a, max_n, s);
+      case 128: return launch_classify_w<KIND, V, 128, TileCfgWide<128>>(m, a, max_n, s);
+      case 256: return launch_classify_w<KIND, V, 256, TileCfgWide<256>>(m, a, max_n, s);
+      case 512: return launch_classify_w<KIND, V, 512, TileCfgWide<512>>(m, a, max_n, s);
+    }
+    return fail(NIQ_ERR_UNSUPPORTED, "width");
+  }
   constexpr bool has_alt = KIND == 0 && (V == 0 || V == 3);
   if constexpr (has_alt) {
     if (variant == 2) {

@@ -9,8 +9,24 @@ namespace {
 struct TreeLayout {
   long long cap;      // max nodes in any level list
   long long nbmax;    // max batches per level
-  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], off_recheck, off_lw, lw_bytes, total;
+  int top_levels;     // levels 0 .. top_levels - 1 are classified in one launch (0: none)
+  long long heap_cap; // heap-index entries per level list
+  size_t off_counters, off_codes, off_bcnt, off_boff, off_lvl[2], off_recheck, off_lw, lw_bytes;
+  size_t off_top_box, off_top_codes, off_heap[2], total;
 };
+
+// The top levels of a tree hold at most 2^k nodes each and would each cost a
+// whole launch of one-tile latency; instead every node of levels
+// 0 .. kTopLevels - 1 (8191 boxes, two tiles per CU) is classified in one
+// launch, and those levels take their codes from it by heap index.
+constexpr int kTopLevels = 13;
+
+int top_levels_for(const niq_eval_cfg* cfg, const niq_tree_params* p) {
+  if (cfg && cfg->recheck_tol > 0.f) return 0;   // the FP64 re-test indexes the level list
+  const int n_levels = p->split_depth >= 0 ? p->split_depth + 1 : 64;
+  const int t = std::min(n_levels, kTopLevels);
+  return t >= 3 ? t : 0;
+}
 
 // Boxes per chunk of the layer-by-layer precision mode inside a tree build.
 constexpr long long kTreeLwChunk = 32768;
@@ -49,18 +65,57 @@ niq_status tree_layout(const niq_model* m, const niq_eval_cfg* cfg, const niq_tr
     L->lw_bytes = 2 * (size_t)nch * 10 * (size_t)std::max(m->host16.wmax, 32) * 2;
   }
   L->off_lw = take(L->lw_bytes);
+  L->top_levels = top_levels_for(cfg, p);
+  const long long n_top = L->top_levels ? (1LL << L->top_levels) - 1 : 0;
+  L->heap_cap = L->top_levels ? std::min(cap, 1LL << (L->top_levels - 1)) : 0;
+  L->off_top_box = take(sizeof(float) * 6 * n_top);
+  L->off_top_codes = take(sizeof(int) * n_top);
+  L->off_heap[0] = take(sizeof(int) * L->heap_cap);
+  L->off_heap[1] = take(sizeof(int) * L->heap_cap);
   L->total = o;
   return NIQ_OK;
+}
+
+// Box of heap node h (root 1; children 2h = lower half, 2h + 1 = upper half
+// along the widest axis, first on ties): the splits of the level loop
+// (k_tree_scatter), replayed from the root in the same float32 operations.
+__global__ void k_top_boxes(float l0, float l1, float l2, float u0, float u1, float u2, const float* root,
+                            int n_top, float* __restrict__ lo, float* __restrict__ up) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (h > n_top) return;
+  float l[3] = {l0, l1, l2}, u[3] = {u0, u1, u2};
+  if (root) {
+    for (int k = 0; k < 3; ++k) { l[k] = root[k]; u[k] = root[3 + k]; }
+  }
+  const int depth = 31 - __clz(h);
+  for (int d = depth - 1; d >= 0; --d) {
+    const float e0 = u[0] - l[0], e1 = u[1] - l[1], e2 = u[2] - l[2];
+    int dim = 0; float em = e0;
+    if (e1 > em) { dim = 1; em = e1; }
+    if (e2 > em) { dim = 2; }
+    const float m = 0.5f * (l[dim] + u[dim]);
+    if ((h >> d) & 1) l[dim] = m; else u[dim] = m;
+  }
+  for (int k = 0; k < 3; ++k) { lo[3 * (h - 1) + k] = l[k]; up[3 * (h - 1) + k] = u[k]; }
+}
+
+// Codes of a top level's list from the one launch over the top of the tree.
+__global__ void k_top_gather(const int* __restrict__ heap, const TreeCounters* __restrict__ C, int level,
+                             const int* __restrict__ top_codes, int* __restrict__ codes) {
+  const int n = C->n[level & 1];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    codes[i] = top_codes[heap[i] - 1];
 }
 
 // Root box from the arguments, or from `root` (6 floats on the device:
 // lower then upper) when it is given, so a captured build can be replayed for
 // any root box.
 __global__ void k_tree_init(TreeCounters* C, float* lo, float* up, float l0, float l1, float l2, float u0,
-                            float u1, float u2, const float* root) {
+                            float u1, float u2, const float* root, int* heap0) {
   if (threadIdx.x == 0) {
     C->n[0] = 1; C->n[1] = 0; C->n_int = 0; C->n_ext = 0; C->n_classified = 0; C->overflow = 0;
     C->recheck_cnt = 0; C->n_recheck = 0;
+    if (heap0) heap0[0] = 1;
     if (root) {
       l0 = root[0]; l1 = root[1]; l2 = root[2];
       u0 = root[3]; u1 = root[4]; u2 = root[5];
@@ -128,9 +183,25 @@ niq_status tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg, const niq_t
   const long long cap = L.cap;
   const int B = p->batch_process_size;
 
+  const int T = L.top_levels;
+  int* heap[2] = {reinterpret_cast<int*>(base + L.off_heap[0]), reinterpret_cast<int*>(base + L.off_heap[1])};
+  int* top_codes = reinterpret_cast<int*>(base + L.off_top_codes);
   hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, C, lvl[0], lvl[0] + 3 * cap, p->lower[0], p->lower[1],
-                     p->lower[2], p->upper[0], p->upper[1], p->upper[2], root_dev);
+                     p->lower[2], p->upper[0], p->upper[1], p->upper[2], root_dev, T ? heap[0] : nullptr);
   HIPCHK(hipGetLastError());
+  if (T) {
+    const int n_top = (1 << T) - 1;
+    float* top_lo = reinterpret_cast<float*>(base + L.off_top_box);
+    float* top_up = top_lo + 3 * (size_t)n_top;
+    hipLaunchKernelGGL(k_top_boxes, dim3((n_top + 255) / 256), dim3(256), 0, s, p->lower[0], p->lower[1],
+                       p->lower[2], p->upper[0], p->upper[1], p->upper[2], root_dev, n_top, top_lo, top_up);
+    HIPCHK(hipGetLastError());
+    ClassifyArgs a{};
+    a.lower = top_lo; a.upper = top_up; a.d = 3; a.n = n_top; a.offset = p->offset;
+    a.codes = top_codes;
+    if (L.lw_bytes) { a.lw_buf = base + L.off_lw; a.lw_bytes = L.lw_bytes; }
+    if (niq_status st = classify_dispatch(m, cfg, 3, a, n_top, s)) return st;
+  }
 
   const int n_levels = p->split_depth >= 0 ? p->split_depth + 1 : 64;
   int level = 0;
@@ -157,7 +228,13 @@ niq_status tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg, const niq_t
       a.recheck_idx = reinterpret_cast<int*>(base + L.off_recheck);
       a.recheck_cap = (int)std::min<long long>(cap, INT_MAX);
     }
-    if (niq_status st = classify_dispatch(m, cfg, 3, a, max_n, s)) return st;
+    if (level < T) {
+      hipLaunchKernelGGL(k_top_gather, dim3((unsigned)((max_n + 255) / 256)), dim3(256), 0, s, heap[level & 1], C,
+                         level, top_codes, codes);
+      HIPCHK(hipGetLastError());
+    } else if (niq_status st = classify_dispatch(m, cfg, 3, a, max_n, s)) {
+      return st;
+    }
 
     const unsigned nb = (unsigned)((max_n + B - 1) / B);
     hipLaunchKernelGGL(k_tree_count, dim3(nb), dim3(256), 0, s, codes, C, level, B, bcnt);
@@ -173,6 +250,11 @@ niq_status tree_enqueue(const niq_model* m, const niq_eval_cfg* cfg, const niq_t
       sa.next_lower = res->unknown_lower;
       sa.next_upper = res->unknown_upper;
       sa.cap_next = res->unknown_capacity;
+    }
+    if (split && level + 1 < T) {
+      sa.heap = heap[level & 1];
+      sa.next_heap = heap[(level + 1) & 1];
+      sa.cap_heap = L.heap_cap;
     }
     if (p->with_interior_nodes) { sa.int_lower = res->interior_lower; sa.int_upper = res->interior_upper; sa.cap_int = res->interior_capacity; }
     if (p->with_exterior_nodes) { sa.ext_lower = res->exterior_lower; sa.ext_upper = res->exterior_upper; sa.cap_ext = res->exterior_capacity; }

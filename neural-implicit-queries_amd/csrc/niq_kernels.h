@@ -21,6 +21,14 @@ template <> struct TileCfgAlt<128> { static constexpr int NB = 64, NW = 8; };
 template <> struct TileCfgAlt<256> { static constexpr int NB = 32, NW = 8; };
 template <> struct TileCfgAlt<512> { static constexpr int NB = 16, NW = 4; };
 
+// Tiles for levels too small to fill the chip: a workgroup's latency bounds
+// them, so its feature tiles are spread over more waves (one or two each).
+template <int WMAX> struct TileCfgWide;
+template <> struct TileCfgWide<64>  { static constexpr int NB = 16, NW = 4; };
+template <> struct TileCfgWide<128> { static constexpr int NB = 16, NW = 8; };
+template <> struct TileCfgWide<256> { static constexpr int NB = 16, NW = 16; };
+template <> struct TileCfgWide<512> { static constexpr int NB = 16, NW = 16; };
+
 constexpr int kLdsLimit = 160 * 1024;
 
 template <int R, int NB, int WMAX>
@@ -418,6 +426,9 @@ struct TreeScatterArgs {
   long long cap_next;
   float* int_lower; float* int_upper; long long cap_int;   // nullable
   float* ext_lower; float* ext_upper; long long cap_ext;   // nullable
+  // heap indices of the nodes (root 1, children 2h and 2h + 1) while the
+  // level lies in the precomputed top of the tree; null below it
+  const int* heap; int* next_heap; long long cap_heap;
 };
 
 __device__ __forceinline__ void store3(float* p, long long i, float x, float y, float z) {
@@ -479,6 +490,11 @@ k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __res
           if (pb < a.cap_next) {
             store3(a.next_lower, pb, dim == 0 ? m0 : l0, dim == 1 ? m1 : l1, dim == 2 ? m2 : l2);
             store3(a.next_upper, pb, u0, u1, u2);
+          }
+          if (a.next_heap) {
+            const int h = a.heap[i];
+            if (pa < a.cap_heap) a.next_heap[pa] = 2 * h;
+            if (pb < a.cap_heap) a.next_heap[pb] = 2 * h + 1;
           }
         } else {
           const long long p = Ub + ru;
