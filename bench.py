@@ -361,6 +361,7 @@ def main():
     torch.cuda.synchronize()
     ray_s = time.perf_counter() - t1
     ray_steps = int(cnt.sum().item())
+    flops_step = models.flops_per_box(dims, rows=5)
 
 
     line = {
@@ -380,7 +381,13 @@ def main():
                    "boxes_per_step": n_boxes // args.steps, "parallelism": "single GPU"},
         "rays": {"rays_per_s": args.ray_res ** 2 / ray_s, "ray_steps_per_s": ray_steps / ray_s,
                  "res": args.ray_res, "seconds": ray_s, "hits": int((hid > 0).sum().item()),
-                 "n_evals": n_evals},
+                 "n_evals": n_evals,
+                 # each step bounds the segment (base, one error row, err) and
+                 # evaluates its two hit-test points: 5 rows through the MLP
+                 "roofline": {"bound": "mfma", "achieved": ray_steps * flops_step / ray_s / 1e12,
+                              "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                              "frac": ray_steps * flops_step / ray_s / 1e12 / PEAK_FP32_TFLOPS,
+                              "kernel": "k_cast_rays (affine_fixed, W=256)", "flops_per_step": flops_step}},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
                      "traffic": C2_CLASSIFY_TRAFFIC_BYTES if n_cls == 1 << 18 else None,

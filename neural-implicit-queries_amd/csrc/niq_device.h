@@ -432,10 +432,66 @@ __device__ __forceinline__ void mlp_forward(const DevModel& M, float* __restrict
         }
       }
     };
-    if (wave + (FTMAX - 1) * NW < nft) contract(std::true_type{});
-    else contract(std::false_type{});
+    // A layer with a single output tile (the last) would leave all waves
+    // but one idle: when it has enough k-blocks they are split into kKSplit
+    // chunks, one per wave, and the partial sums added in chunk order.  The
+    // chunking depends on the model only, not on the tile shape, so every
+    // tile shape gives the same bits.
+    constexpr int kKSplit = 4;
+    const bool ksplit = (nft == 1) && nkb >= 2 * kKSplit;
+    if (ksplit) {
+      if (wave < kKSplit) {
+        const int kb0 = wave * nkb / kKSplit, kb1 = (wave + 1) * nkb / kKSplit;
+        floatx4 a_nx = {0.f, 0.f, 0.f, 0.f};
+        if (kb0 < kb1) a_nx = Ly.Wp[(size_t)kb0 * 64 + lane];
+        for (int kb = kb0; kb < kb1; ++kb) {
+          const floatx4 a = a_nx;
+          floatx4 bx[R][BT];
+#pragma unroll
+          for (int p = 0; p < R; ++p)
+#pragma unroll
+            for (int t = 0; t < BT; ++t)
+              bx[p][t] = *reinterpret_cast<const floatx4*>(&X[xoff<NB, WMAX>(p, t * 16 + ln, kb * 16 + lg * 4)]);
+          if (kb + 1 < kb1) a_nx = Ly.Wp[(size_t)(kb + 1) * 64 + lane];
+#pragma unroll
+          for (int p = 0; p < R; ++p)
+#pragma unroll
+            for (int t = 0; t < BT; ++t)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const float av = PL::absw(p) ? fabsf(a[j]) : a[j];
+                acc[0][p][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bx[p][t][j], acc[0][p][t], 0, 0, 0);
+              }
+        }
+      }
+    } else if (wave + (FTMAX - 1) * NW < nft) {
+      contract(std::true_type{});
+    } else {
+      contract(std::false_type{});
+    }
 
     __syncthreads();  // every wave has consumed X
+
+    if (ksplit) {
+      // partial sums through LDS (X is free now); wave 0 adds them in order
+      floatx4* Pp = reinterpret_cast<floatx4*>(X);
+      if (wave > 0 && wave < kKSplit) {
+#pragma unroll
+        for (int p = 0; p < R; ++p)
+#pragma unroll
+          for (int t = 0; t < BT; ++t) Pp[(((wave - 1) * R + p) * BT + t) * 64 + lane] = acc[0][p][t];
+      }
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int w = 1; w < kKSplit; ++w)
+#pragma unroll
+          for (int p = 0; p < R; ++p)
+#pragma unroll
+            for (int t = 0; t < BT; ++t) acc[0][p][t] += Pp[(((w - 1) * R + p) * BT + t) * 64 + lane];
+      }
+      __syncthreads();  // partials read before the caller reuses X
+    }
 
 #pragma unroll
     for (int f = 0; f < FTMAX; ++f) {
