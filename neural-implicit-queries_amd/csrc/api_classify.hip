@@ -10,7 +10,25 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
   constexpr int NB = T::NB, NW = T::NW;
   constexpr int R = Planes<KIND, V, 0>::R;
   if constexpr (!Fits<R, NB, W>::value) {
-    return fail(NIQ_ERR_UNSUPPORTED, "range form does not fit in LDS at this width");
+    // the planes live in global memory instead (stream-ordered scratch, one
+    // slice per workgroup; the grid is capped to bound it)
+    auto k = k_classify<KIND, V, NB, NW, W, true>;
+    const long long tiles = (max_n + NB - 1) / NB;
+    if (tiles == 0) return NIQ_OK;
+    const unsigned g = grid_for(tiles, std::min<long long>(persistent_blocks(k, NW * 64), 2LL * num_cus()));
+    ClassifyArgs b = a;
+    HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&b.gx), sizeof(float) * (size_t)R * NB * W * g, s));
+    hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, b);
+    const hipError_t e = hipGetLastError();
+    HIPCHK(hipFreeAsync(b.gx, s));
+    if (e != hipSuccess) return fail(NIQ_ERR_HIP, std::string("k_classify (global planes): ") + hipGetErrorString(e));
+    if (a.recheck_tol > 0.f) {
+      auto k64 = k_recheck_fp64<KIND, V, W>;
+      const unsigned g64 = grid_for(std::min<long long>(max_n, a.recheck_cap), persistent_blocks(k64, 256));
+      hipLaunchKernelGGL(k64, dim3(g64), dim3(256), 0, s, m->dev, a);
+      HIPCHK(hipGetLastError());
+    }
+    return NIQ_OK;
   } else {
     auto k = k_classify<KIND, V, NB, NW, W>;
     const long long tiles = (max_n + NB - 1) / NB;
@@ -144,6 +162,17 @@ niq_status launch_points_w(const niq_model* m, const float* x, int d, long long 
 }  // namespace
 
 namespace niqi {
+bool classify_needs_scratch(const niq_model* m, const niq_eval_cfg* cfg, int nvec) {
+  const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
+  const int prec = cfg ? cfg->precision : NIQ_PREC_FP32;
+  if (prec != NIQ_PREC_FP32) return false;
+  int R = 0;
+  if (mode == NIQ_MODE_SLOPE_INTERVAL) R = 1 + 2 * nvec;
+  else if (mode == NIQ_MODE_AFFINE_FIXED) R = nvec + 2;
+  else return false;
+  return (size_t)R * 16 * (size_t)m->wt * 4 > (size_t)kLdsLimit;
+}
+
 niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, int nvec, ClassifyArgs a,
                                  long long n, hipStream_t s) {
   const float tol = cfg ? cfg->recheck_tol : 0.f;

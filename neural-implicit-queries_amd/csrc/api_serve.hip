@@ -270,7 +270,7 @@ void Worker::run_tree(Request* r) {
   const int mode = r->cfg.mode;
   const bool capturable = r->tp.split_depth >= 0 && r->tp.node_terminate_thresh <= 0 &&
                           mode != NIQ_MODE_AFFINE_ALL && mode != NIQ_MODE_AFFINE_TRUNCATE &&
-                          mode != NIQ_MODE_AFFINE_APPEND;
+                          mode != NIQ_MODE_AFFINE_APPEND && !classify_needs_scratch(r->model, &r->cfg, 3);
   TreeGraph* g = capturable ? graph_for(r) : nullptr;
   niq_status st = NIQ_OK;
   auto hipfail = [&](hipError_t e, const char* what) {

@@ -318,6 +318,8 @@ niq_status tree_enqueue_async(const niq_model* m, const niq_eval_cfg* cfg, const
   const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
   if (mode == NIQ_MODE_AFFINE_ALL || mode == NIQ_MODE_AFFINE_TRUNCATE || mode == NIQ_MODE_AFFINE_APPEND)
     return fail(NIQ_ERR_UNSUPPORTED, "niq_tree_enqueue runs the fused-kernel modes only");
+  if (m && classify_needs_scratch(m, cfg, 3))
+    return fail(NIQ_ERR_UNSUPPORTED, "niq_tree_enqueue: this form needs per-call scratch (planes larger than LDS)");
   return tree_enqueue(m, cfg, p, ws, ws_bytes, res, stream, false, nullptr, counters_offset, root_dev);
 }
 

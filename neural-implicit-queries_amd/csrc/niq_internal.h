@@ -132,6 +132,9 @@ inline niq_status check_model(const niq_model* m, int d) {
 // by (mode, precision, box vectors).  Defined in api_classify.hip.
 niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec, const ClassifyArgs& a,
                              long long max_n, hipStream_t s);
+// Whether a classification of `nvec` box vectors keeps its planes in global
+// scratch (allocated per call) because they exceed LDS.
+bool classify_needs_scratch(const niq_model* m, const niq_eval_cfg* cfg, int nvec);
 // The precision-mode launches (api_classify_bf16.hip) and the growing-row
 // modes (api_classify_rows.hip).
 niq_status classify_bf16(const niq_model* m, const niq_eval_cfg* cfg, int nvec, const ClassifyArgs& a,

@@ -65,6 +65,8 @@ struct ClassifyArgs {
   void* lw_buf;            // precision mode, layer by layer: activation scratch
   size_t lw_bytes;         //   (null: allocated per call, stream ordered)
   float lipschitz;         // weak-SDF test: Lipschitz bound of f
+  float* gx;               // forms too large for LDS: activation planes in global
+                           //   memory, R * NB * WMAX floats per workgroup
 };
 
 // Weak-SDF sign code (src/sdf.py:42-48): UNKNOWN where the value at the centre
@@ -112,12 +114,19 @@ __device__ __forceinline__ void fill_box_inputs(float* X, const ClassifyArgs& a,
   }
 }
 
-template <int KIND, int V, int NB, int NW, int WMAX>
-__global__ void __launch_bounds__(NW * 64, (LdsOcc<Planes<KIND, V, 0>::R, NB, NW, WMAX>::value))
+template <int KIND, int V, int NB, int NW, int WMAX, bool GX = false>
+__global__ void __launch_bounds__(NW * 64, (GX ? 2 : LdsOcc<Planes<KIND, V, 0>::R, NB, NW, WMAX>::value))
 k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
   using PL = Planes<KIND, V, 0>;
   constexpr int NT = NW * 64;
-  __shared__ float X[PL::R * NB * WMAX];
+  float* X;
+  if constexpr (GX) {
+    // this workgroup's slice of the global activation planes (L2-resident)
+    X = a.gx + (size_t)blockIdx.x * (PL::R * NB * WMAX);
+  } else {
+    __shared__ float Xs[PL::R * NB * WMAX];
+    X = Xs;
+  }
   const long long n = a.n_dev ? (long long)(*a.n_dev) : a.n;
   const long long ntiles = (n + NB - 1) / NB;
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {

@@ -86,8 +86,6 @@ def test_points_match_oracle(name):
 @pytest.mark.parametrize("mode", ["interval", "affine_fixed", "slope_interval"])
 def test_classify_matches_oracle(name, mode):
     note(f"[classify] {name} {mode}")
-    if mode == "slope_interval" and name == "c4_relu_512x8":
-        pytest.skip("slope_interval at width 512 needs 7 planes x 16 boxes x 512 x 4 B > 160 KB of LDS")
     p = model(name)
     func, _ = implicit_mlp_utils.implicit_from_params(p, mode)
     lo, up = boxes(1500 if name != "c4_relu_512x8" else 600, seed=1, scale=0.5 if name not in ("fox", "bunny") else 0.15)
@@ -119,6 +117,33 @@ def test_classify_matches_oracle(name, mode):
     false_empty = (c2 != 0) & (c2 != c64)
     note(f"  with FP64 re-test: {false_empty.sum()} false-empty, {(c2 != codes).sum()} codes changed")
     assert not np.any(false_empty)
+
+
+def test_slope_interval_wide_model_uses_global_planes():
+    """slope_interval on the 512-wide net keeps its 7 planes (224 KB per tile)
+    in global memory: bounds as the oracle's on a launch of full tiles, and a
+    small tree identical to the oracle's."""
+    note("[slope_interval c4 global planes]")
+    p = model("c4_relu_512x8")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "slope_interval")
+    lo, up = boxes(10000, seed=2, scale=0.05)
+    codes, bounds = func.classify_box(p, torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda"),
+                                      return_bounds=True)
+    r_lo, r_hi = ofuncs.OracleFunc(p, "slope_interval", np.float32).bounds_box(lo, up)
+    r64_lo, r64_hi = ofuncs.OracleFunc(p, "slope_interval", np.float64).bounds_box(lo, up)
+    b = bounds.cpu().numpy()
+    scale = np.max(np.abs(np.r_[r_lo, r_hi]))
+    tol_lo = 1e-5 * scale + 2 * np.abs(r_lo - r64_lo)
+    tol_hi = 1e-5 * scale + 2 * np.abs(r_hi - r64_hi)
+    assert np.all(np.abs(b[:, 0] - r_lo) <= tol_lo) and np.all(np.abs(b[:, 1] - r_hi) <= tol_hi)
+    out, st = kd_tree.build_tree(func, p, LO, UP, split_depth=9, with_interior_nodes=True, with_exterior_nodes=True)
+    ref = otree.construct_uniform_unknown_levelset_tree(ofuncs.OracleFunc(p, "slope_interval"), LO, UP, split_depth=9,
+                                                        with_interior_nodes=True, with_exterior_nodes=True)
+    assert st["n_classified"] == ref["stats"]["n_classified"]
+    for key in ("unknown", "interior", "exterior"):
+        v = out[f"{key}_node_valid"].cpu().numpy()
+        np.testing.assert_array_equal(v, ref[f"{key}_node_valid"])
+        np.testing.assert_array_equal(out[f"{key}_node_lower"].cpu().numpy()[v], ref[f"{key}_node_lower"][v])
 
 
 def test_general_boxes_match_oracle():
