@@ -53,9 +53,9 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-sized tensor (a plain copy kernel of known bytes, "
                          "to scale the FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--serve-requests", type=int, default=120,
+    ap.add_argument("--serve-requests", type=int, default=90,
                     help="requests in the serving trace (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=5.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=3.0, help="Poisson arrival rate, requests/s")
     return ap.parse_args()
 
 
