@@ -331,6 +331,12 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
           const bf16x8 x2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(1, b, k0)]);
           const bf16x8 x3 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(2, b, k0)]);
           const bf16x8 xe = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(PB::E, b, k0)]);
+          bf16x8 xa[V > 0 ? V : 1][2];   // error-term rows, read once for all feature tiles
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            xa[i][0] = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k0)]);
+            xa[i][1] = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k0)]);
+          }
 #pragma unroll
           for (int f = 0; f < FTMAX; ++f) {
             const int ft = wave + f * NW;
@@ -355,8 +361,8 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
             }
 #pragma unroll
             for (int i = 0; i < V; ++i) {
-              const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(3 + 2 * i, b, k0)]);
-              const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(&Xs[boff16<NB, WMAX>(4 + 2 * i, b, k0)]);
+              const bf16x8 a1 = xa[i][0];
+              const bf16x8 a2 = xa[i][1];
               floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a2, z4, 0, 0, 0);
               c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2, a1, c, 0, 0, 0);
               c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a1, c, 0, 0, 0);
