@@ -298,14 +298,17 @@ niq_status niq_bulk_sums_nodes(const niq_model* m, const float* unknown_lower, c
 niq_status niq_hit_positions(const float* roots, const float* dirs, const float* t, int64_t n,
                              float* out, void* stream);
 
-/* Replaces outward_normals with method='finite_differences' (src/render.py:53-90):
- * per pixel the normalised gradient of function hit_id (1..n_models) from the
- * 4 tetrahedron samples hit_pos + (+-eps, +-eps, +-eps); (0, 0, 0) where
- * hit_id matches no function.  hit_pos, out_normals (n, 3) float32; hit_ids
- * (n,) int32.  Asynchronous on `stream`. */
+/* Replaces outward_normals (src/render.py:53-90): per pixel the normalised
+ * gradient of function hit_id (1..n_models); (0, 0, 0) where hit_id matches
+ * no function.  method NIQ_NORMALS_FINITE_DIFFERENCES: from the 4 tetrahedron
+ * samples hit_pos + (+-eps, +-eps, +-eps); NIQ_NORMALS_AUTODIFF: the exact
+ * forward-mode gradient (jax.jacfwd), eps unused.  hit_pos, out_normals (n, 3)
+ * float32; hit_ids (n,) int32.  Asynchronous on `stream`. */
+#define NIQ_NORMALS_FINITE_DIFFERENCES 0
+#define NIQ_NORMALS_AUTODIFF 1
 niq_status niq_outward_normals(const niq_model* const* models, int n_models, const float* hit_pos,
-                               const int32_t* hit_ids, int64_t n, float eps, float* out_normals,
-                               void* stream);
+                               const int32_t* hit_ids, int64_t n, float eps, int method,
+                               float* out_normals, void* stream);
 
 #define NIQ_SHADE_NORMAL 0
 #define NIQ_SHADE_MATCAP 1

@@ -26,6 +26,19 @@ niq_status launch_normal_eval(const niq_model* m, const int* list, const int* co
   return NIQ_OK;
 }
 
+template <int W>
+niq_status launch_grad_normals(const niq_model* m, const int* list, const int* count, long long max_n,
+                               const float* hit_pos, float* normals, hipStream_t s) {
+  constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
+  auto k = k_grad_normals<NB, NW, W>;
+  const long long tiles = (max_n + NB - 1) / NB;
+  if (tiles == 0) return NIQ_OK;
+  const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+  hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, list, count, hit_pos, normals);
+  HIPCHK(hipGetLastError());
+  return NIQ_OK;
+}
+
 }  // namespace
 
 extern "C" niq_status niq_hit_positions(const float* roots, const float* dirs, const float* t, int64_t n,
@@ -39,9 +52,11 @@ extern "C" niq_status niq_hit_positions(const float* roots, const float* dirs, c
 }
 
 extern "C" niq_status niq_outward_normals(const niq_model* const* models, int n_models, const float* hit_pos,
-                                          const int32_t* hit_ids, int64_t n, float eps, float* out_normals,
-                                          void* stream) {
+                                          const int32_t* hit_ids, int64_t n, float eps, int method,
+                                          float* out_normals, void* stream) {
   if (!models || n_models < 1 || n_models > 8) return fail(NIQ_ERR_INVALID, "need 1..8 models");
+  if (method != NIQ_NORMALS_FINITE_DIFFERENCES && method != NIQ_NORMALS_AUTODIFF)
+    return fail(NIQ_ERR_INVALID, "unrecognized method");
   for (int i = 0; i < n_models; ++i)
     if (niq_status st = check_model(models[i], 3)) return st;
   if (n < 0 || n >= (1LL << 31)) return fail(NIQ_ERR_INVALID, "need 0 <= n < 2^31");
@@ -62,7 +77,17 @@ extern "C" niq_status niq_outward_normals(const niq_model* const* models, int n_
   if (st == NIQ_OK)
     hipLaunchKernelGGL(k_hit_lists, dim3(elem_grid(n)), dim3(256), 0, s, hit_ids, (long long)n, n_models,
                        (long long)n, lists, counts, out_normals);
-  for (int f = 0; f < n_models && st == NIQ_OK; ++f) {
+  for (int f = 0; f < n_models && st == NIQ_OK && method == NIQ_NORMALS_AUTODIFF; ++f) {
+    const int* list = lists + (size_t)f * n;
+    switch (models[f]->wt) {
+      case 64: st = launch_grad_normals<64>(models[f], list, counts + f, n, hit_pos, out_normals, s); break;
+      case 128: st = launch_grad_normals<128>(models[f], list, counts + f, n, hit_pos, out_normals, s); break;
+      case 256: st = launch_grad_normals<256>(models[f], list, counts + f, n, hit_pos, out_normals, s); break;
+      case 512: st = launch_grad_normals<512>(models[f], list, counts + f, n, hit_pos, out_normals, s); break;
+      default: st = fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+  }
+  for (int f = 0; f < n_models && st == NIQ_OK && method == NIQ_NORMALS_FINITE_DIFFERENCES; ++f) {
     const int* list = lists + (size_t)f * n;
     switch (models[f]->wt) {
       case 64: st = launch_normal_eval<64>(models[f], list, counts + f, n, hit_pos, eps, vals, s); break;

@@ -54,6 +54,7 @@ template <> struct Fm<float> {
   static __device__ __forceinline__ float expm1(float x) { return expm1f(x); }
   static __device__ __forceinline__ float log(float x) { return logf(x); }
   static __device__ __forceinline__ float sin(float x) { return sinf(x); }
+  static __device__ __forceinline__ float cos(float x) { return cosf(x); }
   static __device__ __forceinline__ float acos(float x) { return acosf(x); }
   static __device__ __forceinline__ float tanh(float x) { return tanhf(x); }
   static __device__ __forceinline__ float atanh(float x) { return atanhf(x); }
@@ -72,6 +73,7 @@ template <> struct Fm<double> {
   static __device__ __forceinline__ double expm1(double x) { return ::expm1(x); }
   static __device__ __forceinline__ double log(double x) { return ::log(x); }
   static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
+  static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
   static __device__ __forceinline__ double acos(double x) { return ::acos(x); }
   static __device__ __forceinline__ double tanh(double x) { return ::tanh(x); }
   static __device__ __forceinline__ double atanh(double x) { return ::atanh(x); }
@@ -108,6 +110,18 @@ template <class T> __device__ __forceinline__ T act_point(int act, T x) {
     case ACT_SIN: return Fm<T>::sin(x);
     case ACT_TANH: return Fm<T>::tanh(x);
     default: return x;
+  }
+}
+
+// Derivative of each activation as jax differentiates it (jax.nn.relu: 0 at
+// 0; jax.nn.elu: exp(x) for x <= 0; sin: cos; tanh: 1 - tanh^2).
+template <class T> __device__ __forceinline__ T act_deriv(int act, T x) {
+  switch (act) {
+    case ACT_RELU: return x > T(0) ? T(1) : T(0);
+    case ACT_ELU: return x > T(0) ? T(1) : Fm<T>::exp(x);
+    case ACT_SIN: return Fm<T>::cos(x);
+    case ACT_TANH: { const T t = Fm<T>::tanh(x); return T(1) - t * t; }
+    default: return T(1);
   }
 }
 
@@ -235,10 +249,11 @@ __device__ __forceinline__ float sign_f(float x) {
 //  kind 0: affine   [base, aff_0..aff_{V-1}, err]      R = V + 2 (interval: V = 0)
 //  kind 1: slope    [primal, c_0..c_{V-1}, w_0..w_{V-1}] R = 1 + 2V
 //  kind 2: none     (points only)
+//  kind 3: tangent  [primal, d/dx_0..d/dx_{V-1}]          R = 1 + V (forward-mode gradient)
 // followed by P point planes.
 template <int KIND, int V, int P>
 struct Planes {
-  static constexpr int RB = KIND == 0 ? V + 2 : (KIND == 1 ? 1 + 2 * V : 0);
+  static constexpr int RB = KIND == 0 ? V + 2 : (KIND == 1 ? 1 + 2 * V : (KIND == 3 ? 1 + V : 0));
   static constexpr int R = RB + P;
   // plane multiplies |W| instead of W
   __device__ static constexpr bool absw(int p) {
@@ -343,6 +358,14 @@ template <class PL, int KIND, int V, class T>
 __device__ __forceinline__ void apply_hidden(int act, T* v) {
   if constexpr (KIND == 0) affine_fixed_rule<V>(act, v);
   else if constexpr (KIND == 1) slope_rule<V>(act, v);
+  else if constexpr (KIND == 3) {
+    if (act != ACT_NONE) {
+      const T d = act_deriv(act, v[0]);
+#pragma unroll
+      for (int i = 1; i <= V; ++i) v[i] = v[i] * d;
+      v[0] = act_point(act, v[0]);
+    }
+  }
 #pragma unroll
   for (int p = PL::RB; p < PL::R; ++p) v[p] = act_point(act, v[p]);
 }

@@ -84,6 +84,38 @@ k_normal_eval(const DevModel* __restrict__ Mp, const int* __restrict__ list, con
   }
 }
 
+// Forward-mode gradient (method 'autodiff', jax.jacfwd of the MLP,
+// src/render.py:62-64), normalised: one box per listed pixel with the primal
+// plane at the hit and three unit tangent planes.
+template <int NB, int NW, int WMAX>
+__global__ void __launch_bounds__(NW * 64)
+k_grad_normals(const DevModel* __restrict__ Mp, const int* __restrict__ list, const int* __restrict__ count,
+               const float* __restrict__ hit_pos, float* __restrict__ normals) {
+  using PL = Planes<3, 3, 0>;
+  constexpr int NT = NW * 64;
+  __shared__ float X[PL::R * NB * WMAX];
+  const long long n = *count;
+  const long long ntiles = (n + NB - 1) / NB;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long i0 = tile * NB;
+    for (int e = threadIdx.x; e < PL::R * NB * 16; e += NT) {
+      const int p = e / (NB * 16), b = (e / 16) % NB, k = e & 15;
+      const long long s = i0 + b;
+      float val = 0.f;
+      if (s < n && k < 3) val = p == 0 ? hit_pos[3 * (long long)list[s] + k] : (p - 1 == k ? 1.f : 0.f);
+      X[xoff<NB, WMAX>(p, b, k)] = val;
+    }
+    mlp_forward<PL, 3, 3, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+      const long long s = i0 + b;
+      if (s >= n) return;
+      const float nrm = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(v[1], v[1]), __fmul_rn(v[2], v[2])), __fmul_rn(v[3], v[3])));
+      const long long pix = list[s];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) normals[3 * pix + k] = v[1 + k] / nrm;
+    });
+  }
+}
+
 // Tetrahedron gradient, normalised (src/render.py:74-79, src/geometry.py:12).
 static __global__ void __launch_bounds__(256)
 k_normal_finish(const int* __restrict__ list, const int* __restrict__ count, const float* __restrict__ vals,

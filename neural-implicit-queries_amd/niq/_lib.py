@@ -148,7 +148,7 @@ def load_library():
                                          ctypes.POINTER(I64), ctypes.POINTER(I64), P, S, P]),
         "niq_bulk_sums_nodes": (I, [P, P, P, I64, P, P, I64, I64, ctypes.c_uint64, P, P]),
         "niq_hit_positions": (I, [P, P, P, I64, P, P]),
-        "niq_outward_normals": (I, [P, I, P, P, I64, F, P, P]),
+        "niq_outward_normals": (I, [P, I, P, P, I64, F, I, P, P]),
         "niq_shade_image": (I, [P, P, P, P, I64, P, P]),
         "niq_serve_create": (I, [I, S, I64, ctypes.POINTER(P)]),
         "niq_serve_destroy": (I, [P]),

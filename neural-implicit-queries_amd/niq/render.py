@@ -84,10 +84,11 @@ def _tuple(x):
 
 def outward_normals(funcs_tuple, params_tuple, hit_pos, hit_ids, eps, method='finite_differences'):
     """src/render.py:82-90 (vmapped outward_normal, :53-80): (N, 3) unit normals
-    of function hit_id at each hit; zeros where hit_id matches none."""
-    if method == 'autodiff':
-        raise NotImplementedError("only the finite_differences normals are provided")
-    if method != 'finite_differences':
+    of function hit_id at each hit (tetrahedron finite differences, or the
+    exact forward-mode gradient for method='autodiff'); zeros where hit_id
+    matches none."""
+    methods = {'finite_differences': 0, 'autodiff': 1}
+    if method not in methods:
         raise ValueError("unrecognized method")
     funcs_tuple, params_tuple = _tuple(funcs_tuple), _tuple(params_tuple)
     if len(funcs_tuple) != len(params_tuple):
@@ -102,7 +103,7 @@ def outward_normals(funcs_tuple, params_tuple, hit_pos, hit_ids, eps, method='fi
     handles = (ctypes.c_void_p * len(models))(*[m.handle.value for m in models])
     out = torch.empty_like(pos)
     _lib.check(lib.niq_outward_normals(handles, len(models), _lib.ptr(pos), _lib.ptr(ids), pos.shape[0],
-                                       float(eps), _lib.ptr(out), _lib.stream_handle()))
+                                       float(eps), methods[method], _lib.ptr(out), _lib.stream_handle()))
     return out
 
 

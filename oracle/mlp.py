@@ -102,3 +102,47 @@ def eval_points(params, x, dtype=np.float32):
         else:
             raise ValueError(f"unsupported op {name}")
     return h
+
+
+def act_deriv(name, x):
+    """Derivatives as jax differentiates the default activations: jax.nn.relu
+    has derivative 0 at 0; jax.nn.elu's is exp(x) for x <= 0; tanh's 1 - tanh^2."""
+    t = x.dtype.type
+    if name == "relu":
+        return np.where(x > 0, t(1), t(0)).astype(x.dtype)
+    if name == "elu":
+        return np.where(x > 0, t(1), np.exp(np.minimum(x, t(0)))).astype(x.dtype)
+    if name == "sin":
+        return np.cos(x)
+    if name == "tanh":
+        return (t(1) - np.tanh(x) ** 2).astype(x.dtype)
+    raise ValueError(name)
+
+
+def eval_points_grad(params, x, dtype=np.float32):
+    """Value and gradient of the MLP at points x (N, d): forward mode, as
+    jax.jacfwd of func_from_spec('default') computes it (src/render.py:62-64)."""
+    h = np.asarray(x, dtype=dtype)
+    N, d = h.shape
+    T = np.broadcast_to(np.eye(d, dtype=dtype), (N, d, d)).copy()   # (N, d tangents, features)
+    for name, args in op_list(params):
+        if name == "dense":
+            A = np.asarray(args["A"], dtype=dtype)
+            h = h @ A
+            T = T @ A
+            if "b" in args:
+                h = h + np.asarray(args["b"], dtype=dtype)
+        elif name in ACTIVATIONS:
+            dv = act_deriv(name, h)
+            T = (T * dv[:, None, :]).astype(dtype)
+            h = act_point(name, h)
+        elif name == "squeeze_last":
+            h = h[..., 0]
+            T = T[..., 0]
+        elif name == "spatial_transformation":
+            R_inv, t_inv = spatial_inverse(args["R"], args["t"], dtype)
+            h = h @ R_inv + t_inv
+            T = T @ R_inv
+        else:
+            raise ValueError(f"unsupported op {name}")
+    return h, T

@@ -3,7 +3,7 @@
 Test infrastructure only (see ``oracle/__init__.py``).
 
 Restates (float32, vectorised over pixels):
-  * outward_normal / outward_normals (finite differences) -- src/render.py:53-90
+  * outward_normal / outward_normals (both methods)       -- src/render.py:53-90
   * shade_image ('normal', 'matcap_color')                -- src/render.py:160-224
   * the compositing of render_image                        -- src/render.py:139-149
   * tonemap_image                                          -- src/render.py:152-158
@@ -56,6 +56,21 @@ def outward_normals(funcs, hit_pos, hit_ids, eps):
         for q in range(4):                    # sum over the 4 samples in order
             grad = (grad + off[q][None, :] * s[:, q:q + 1]).astype(f32)
         out[sel] = normalize(grad)
+    return out
+
+
+def outward_normals_autodiff(funcs, hit_pos, hit_ids):
+    """src/render.py:62-64 (method='autodiff'): the normalised forward-mode
+    gradient of function hit_id; zeros where no function matches."""
+    from .mlp import eval_points_grad
+    hit_pos = np.asarray(hit_pos, f32)
+    hit_ids = np.asarray(hit_ids)
+    out = np.zeros_like(hit_pos)
+    for fid, func in enumerate(funcs, start=1):
+        sel = hit_ids == fid
+        if sel.any():
+            _, g = eval_points_grad(func.params, hit_pos[sel], f32)
+            out[sel] = normalize(g)
     return out
 
 

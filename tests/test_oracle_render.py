@@ -91,3 +91,28 @@ def test_normal_shading_and_tonemap():
                        rtol=1e-5)
     img = orender.composite(np.zeros((3, 3), f32), np.array([0, 1, 2]))
     assert np.array_equal(img, [[1, 1, 1], [0, 0, 0], [0, 0, 0]])
+
+
+def test_autodiff_normals_match_float64_gradient():
+    """Forward-mode gradients against float64 central differences (smooth net)
+    and a plane's normal exactly; relu's derivative at a kink is jax's 0."""
+    from oracle import mlp as omlp
+    p = models.random_mlp([32, 32], activation="tanh", seed=6)
+    f = ofuncs.OracleFunc(p)
+    f64 = ofuncs.OracleFunc(p, dtype=np.float64)
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-0.8, 0.8, (200, 3)).astype(f32)
+    nr = orender.outward_normals_autodiff((f,), pos, np.ones(200, int))
+    h = 1e-6
+    g = np.stack([(f64(pos + h * e) - f64(pos - h * e)).reshape(-1) / (2 * h) for e in np.eye(3)], 1)
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    assert np.all(np.sum(g * nr, axis=1) > 0.99999)
+    n1 = np.array([0.36, -0.48, 0.8])
+    pl = ofuncs.OracleFunc(plane_mlp(n1, 0.1))
+    assert np.allclose(orender.outward_normals_autodiff((pl,), pos, np.ones(200, int)), n1, atol=1e-6)
+    # the value part equals plain evaluation
+    v, _ = omlp.eval_points_grad(p, pos)
+    assert np.array_equal(v, omlp.eval_points(p, pos))
+    x = np.zeros(4, f32)
+    assert np.array_equal(omlp.act_deriv("relu", x), np.zeros(4, f32))
+    assert np.array_equal(omlp.act_deriv("elu", x), np.ones(4, f32))
