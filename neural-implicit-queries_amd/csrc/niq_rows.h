@@ -85,11 +85,13 @@ k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
   __shared__ float s_mag[2 * WMAX + 64];    // row magnitudes of the candidates
   __shared__ unsigned char s_taken[2 * WMAX + 64];
   __shared__ float s_rv[NT];
+  __shared__ float s_pe[NT];
   __shared__ int s_ri[NT];
   __shared__ int s_keep[64];
   const ClassifyArgs& a = ra.c;
   const DevModel& M = *Mp;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // scalar tile tests
   const int ln = lane & 15, lg = lane >> 4;
   const long long n = a.n_dev ? (long long)(*a.n_dev) : a.n;
   float* buf[2] = {ra.scratch + (size_t)blockIdx.x * 2 * ra.max_rows * WMAX,
@@ -129,26 +131,72 @@ k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
       const bool last = (l == M.n_layers - 1);
       float* Ein = buf[cur];
       float* Eout = buf[cur ^ 1];
-      // base . A + b and err . |A| (sequential over k)
-      for (int f = threadIdx.x; f < Ly.out_pad; f += NT) {
-        float acc = 0.f, ea = 0.f;
-        for (int k = 0; k < Ly.in_pad; ++k) {
-          const float w = Ly.W[(size_t)k * Ly.out_pad + f];
-          acc = acc + s_base[k] * w;
-          if (MODE != ROWS_ALL) ea = ea + s_err[k] * fabsf(w);
+      // base . A + b and err . |A|: sequential over k per feature, or, for a
+      // layer narrower than the workgroup, over G ranges of k joined in order
+      if (Ly.out_pad >= NT) {
+        for (int f = threadIdx.x; f < Ly.out_pad; f += NT) {
+          float acc = 0.f, ea = 0.f;
+#pragma unroll 8
+          for (int k = 0; k < Ly.in_pad; ++k) {
+            const float w = Ly.W[(size_t)k * Ly.out_pad + f];
+            acc = acc + s_base[k] * w;
+            if (MODE != ROWS_ALL) ea = ea + s_err[k] * fabsf(w);
+          }
+          s_nb[f] = acc + Ly.bias[f];
+          s_ne[f] = ea;
         }
-        s_nb[f] = acc + Ly.bias[f];
-        s_ne[f] = ea;
+      } else {
+        const int G = NT / Ly.out_pad;
+        const int grp = (int)threadIdx.x / Ly.out_pad, f = (int)threadIdx.x % Ly.out_pad;
+        const int per = (Ly.in_pad + G - 1) / G;
+        if (grp < G) {
+          float acc = 0.f, ea = 0.f;
+          const int k0 = grp * per, k1 = min(Ly.in_pad, k0 + per);
+#pragma unroll 8
+          for (int k = k0; k < k1; ++k) {
+            const float w = Ly.W[(size_t)k * Ly.out_pad + f];
+            acc = acc + s_base[k] * w;
+            if (MODE != ROWS_ALL) ea = ea + s_err[k] * fabsf(w);
+          }
+          s_rv[threadIdx.x] = acc;
+          s_pe[threadIdx.x] = ea;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < Ly.out_pad) {
+          float acc = 0.f, ea = 0.f;
+          for (int g2 = 0; g2 < G; ++g2) {
+            acc = acc + s_rv[g2 * Ly.out_pad + threadIdx.x];
+            ea = ea + s_pe[g2 * Ly.out_pad + threadIdx.x];
+          }
+          s_nb[threadIdx.x] = acc + Ly.bias[threadIdx.x];
+          s_ne[threadIdx.x] = ea;
+        }
       }
       floatx4 csum[FTMAX];
 #pragma unroll
       for (int f = 0; f < FTMAX; ++f) csum[f] = floatx4{0.f, 0.f, 0.f, 0.f};
       for (long long r0 = 0; r0 < n_stored; r0 += CH) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < CH * Ly.in_pad; e += NT) {
+        // the chunk's rows: every 16-byte load issued before any is used
+        constexpr int NL = CH * WMAX / 4 / NT;
+        floatx4 ld[NL];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          const int e = 4 * ((int)threadIdx.x + j * NT);
           const int rr = e / Ly.in_pad, k = e % Ly.in_pad;
           const long long r = r0 + rr;
-          X[xoff<CH, WMAX>(0, rr, k)] = (r < n_stored) ? s_alpha[k] * Ein[r * WMAX + k] : 0.f;
+          ld[j] = (e < CH * Ly.in_pad && r < n_stored) ? *reinterpret_cast<const floatx4*>(&Ein[r * WMAX + k])
+                                                       : floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          const int e = 4 * ((int)threadIdx.x + j * NT);
+          if (e >= CH * Ly.in_pad) continue;
+          const int rr = e / Ly.in_pad, k = e % Ly.in_pad;
+          floatx4 v;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[q] = s_alpha[k + q] * ld[j][q];
+          *reinterpret_cast<floatx4*>(&X[xoff<CH, WMAX>(0, rr, k)]) = v;
         }
         __syncthreads();
         floatx4 acc[FTMAX][BT];
@@ -156,12 +204,23 @@ k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
         for (int f = 0; f < FTMAX; ++f)
 #pragma unroll
           for (int t = 0; t < BT; ++t) acc[f][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        // weights one k-block ahead of the MFMAs
+        floatx4 w_nx[FTMAX];
+#pragma unroll
+        for (int f = 0; f < FTMAX; ++f) {
+          const int ft = wave + f * NW;
+          if (ft < nft) w_nx[f] = Ly.Wp[(size_t)(ft * nkb) * 64 + lane];
+        }
         for (int kb = 0; kb < nkb; ++kb) {
           floatx4 w[FTMAX];
 #pragma unroll
-          for (int f = 0; f < FTMAX; ++f) {
-            const int ft = wave + f * NW;
-            if (ft < nft) w[f] = Ly.Wp[(size_t)(ft * nkb + kb) * 64 + lane];
+          for (int f = 0; f < FTMAX; ++f) w[f] = w_nx[f];
+          if (kb + 1 < nkb) {
+#pragma unroll
+            for (int f = 0; f < FTMAX; ++f) {
+              const int ft = wave + f * NW;
+              if (ft < nft) w_nx[f] = Ly.Wp[(size_t)(ft * nkb + kb + 1) * 64 + lane];
+            }
           }
 #pragma unroll
           for (int t = 0; t < BT; ++t) {
@@ -213,14 +272,40 @@ k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
       __syncthreads();
       // added rows: scale_r * A[idx_r, :], appended after the stored rows;
       // radius = sum |E| + err (src/affine.py:85-91)
-      for (int f = threadIdx.x; f < Ly.out_pad; f += NT) {
-        float dsum = 0.f;
-        for (int r = 0; r < n_new; ++r) {
-          const float v = s_scale[r] * Ly.W[(size_t)s_idx[r] * Ly.out_pad + f];
-          if (!last) Eout[(n_stored + r) * WMAX + f] = v;
-          dsum = dsum + fabsf(v);
+      if (Ly.out_pad >= NT) {
+        for (int f = threadIdx.x; f < Ly.out_pad; f += NT) {
+          float dsum = 0.f;
+#pragma unroll 8
+          for (int r = 0; r < n_new; ++r) {
+            const float v = s_scale[r] * Ly.W[(size_t)s_idx[r] * Ly.out_pad + f];
+            if (!last) Eout[(n_stored + r) * WMAX + f] = v;
+            dsum = dsum + fabsf(v);
+          }
+          s_rad[f] = (s_rad[f] + dsum) + s_ne[f];
         }
-        s_rad[f] = (s_rad[f] + dsum) + s_ne[f];
+      } else {
+        // narrower than the workgroup: G groups of threads split the added
+        // rows, and the partial sums join in group order
+        const int G = NT / Ly.out_pad;
+        const int grp = (int)threadIdx.x / Ly.out_pad, f = (int)threadIdx.x % Ly.out_pad;
+        const int per = (n_new + G - 1) / G;
+        if (grp < G) {
+          float dsum = 0.f;
+          const int ra0 = grp * per, ra1 = min(n_new, ra0 + per);
+#pragma unroll 8
+          for (int r = ra0; r < ra1; ++r) {
+            const float v = s_scale[r] * Ly.W[(size_t)s_idx[r] * Ly.out_pad + f];
+            if (!last) Eout[(n_stored + r) * WMAX + f] = v;
+            dsum = dsum + fabsf(v);
+          }
+          s_rv[threadIdx.x] = dsum;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < Ly.out_pad) {
+          float dsum = 0.f;
+          for (int g2 = 0; g2 < G; ++g2) dsum = dsum + s_rv[g2 * Ly.out_pad + threadIdx.x];
+          s_rad[threadIdx.x] = (s_rad[threadIdx.x] + dsum) + s_ne[threadIdx.x];
+        }
       }
       __syncthreads();
       const long long n_out = n_stored + n_new;
