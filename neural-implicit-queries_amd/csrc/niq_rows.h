@@ -65,8 +65,12 @@ __device__ __forceinline__ int block_argmax(const float* vals, const unsigned ch
   return r;
 }
 
+// Register budget: the waves per SIMD that LDS admits (four 41 KB workgroups
+// per CU at width <= 128).
+template <int WMAX> struct RowsOcc { static constexpr int value = WMAX <= 128 ? 4 : 1; };
+
 template <int WMAX, int MODE>
-__global__ void __launch_bounds__(RowsCfg<WMAX>::NW * 64)
+__global__ void __launch_bounds__(RowsCfg<WMAX>::NW * 64, RowsOcc<WMAX>::value)
 k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
   constexpr int CH = RowsCfg<WMAX>::CH, NW = RowsCfg<WMAX>::NW;
   constexpr int NT = NW * 64;
