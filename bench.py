@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--precision-ray-res", type=int, default=512,
                     help="side of the ray cast timed in FP32 and BF16 (0 skips it)")
     ap.add_argument("--no-concurrent", action="store_true", help="skip the 64 concurrent trees")
+    ap.add_argument("--no-next", action="store_true", help="skip frustum, sampling, bulk, render, intersection, sdf")
     ap.add_argument("--no-c3", action="store_true", help="skip the affine_all tree")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-sized tensor (a plain copy kernel of known bytes, "
@@ -214,6 +215,61 @@ def serving_trace(n_req, rate, seed=0):
             "requests_per_s": n_req / wall, "wall_s": wall, "latency_ms": lat,
             "rays_merged_requests": merged, "arrival_rate_per_s": rate, "trees_from_graphs": replays,
             "models": [m[0] for m in resident]}
+
+
+def next_rows():
+    """The queries beyond the tree and the ray cast, on the reference's trained
+    fox SDF (tests/golden/fox.npz): frustum against per-ray casting at 1024^2,
+    level-set sampling, bulk properties, a matcap render, an intersection
+    search and a tree in the sdf mode.  Each is timed once after a warm-up."""
+    import torch
+    from niq import implicit_mlp_utils, kd_tree, queries, render, models, mlp as nmlp
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    fox = nmlp.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "fox.npz"))
+    ff, _ = implicit_mlp_utils.implicit_from_params(fox, "affine_fixed")
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    eye = np.array([0., 0., 3.], np.float32)
+    look, upd, left = render.look_at(eye)
+    res = 1024
+    out = {"model": "fox (the reference's sample SDF)"}
+    sf, rf = timed(lambda: queries.cast_rays_frustum((ff,), (fox,), (eye, look, upd, left, 60.0, 60.0, res, res), opts))
+    roots, dirs = render.generate_camera_rays(eye, look, upd, res=res, fov_deg=60.)
+    sr, rr = timed(lambda: queries.cast_rays((ff,), (fox,), roots, dirs, opts))
+    out["frustum_1024"] = {"seconds": sf, "rays_per_s": res * res / sf, "hits": int((rf[1] > 0).sum().item()),
+                           "per_ray_seconds": sr, "per_ray_rays_per_s": res * res / sr,
+                           "per_ray_hits": int((rr[1] > 0).sum().item()), "speedup_over_per_ray": sr / sf}
+    n_samp = 1000000
+    ss, _ = timed(lambda: kd_tree.sample_surface(ff, fox, lo, up, n_samp, 0.005, 1))
+    out["sample_surface"] = {"samples": n_samp, "width": 0.005, "seconds": ss, "samples_per_s": n_samp / ss,
+                             "rounds": kd_tree.sample_surface.last_rounds}
+    sb, (mass, _) = timed(lambda: kd_tree.bulk_properties(ff, fox, lo, up, 1))
+    out["bulk_properties"] = {"samples": 1000000, "seconds": sb, "mass": float(mass)}
+    mats = tuple(np.random.default_rng(0).uniform(0, 1, (256, 256, 3)).astype(np.float32) for _ in range(4))
+    e2 = np.array([0., 0.3, 2.5], np.float32)
+    l2, u2, f2 = render.look_at(e2)
+    sx, _ = timed(lambda: render.render_image((ff,), (fox,), e2, l2, u2, f2, res, 45.0, True, dict(opts, n_side_init=16),
+                                              shading="matcap_color", matcaps=mats, tonemap=True))
+    out["render_image_frustum_matcap_1024"] = {"seconds": sx, "frames_per_s": 1.0 / sx}
+    a = models.random_mlp([32, 32], seed=11)
+    b = models.random_mlp([48, 24], activation="tanh", seed=12)
+    fa, _ = implicit_mlp_utils.implicit_from_params(a, "affine_fixed")
+    fb, _ = implicit_mlp_utils.implicit_from_params(b, "affine_fixed")
+    si, _ = timed(lambda: kd_tree.find_any_intersection((fa, fb), (a, b), lo, up, 0.001))
+    out["find_any_intersection"] = {"seconds": si, "nodes": kd_tree.find_any_intersection.last_n_processed}
+    fs, _ = implicit_mlp_utils.implicit_from_params(fox, "sdf")
+    st, (_, stats) = timed(lambda: kd_tree.build_tree(fs, fox, lo, up, split_depth=21))
+    out["sdf_tree_21"] = {"seconds": st, "boxes": stats["n_classified"], "boxes_per_s": stats["n_classified"] / st}
+    return out
 
 
 def c3_affine_all(depth=21):
@@ -402,6 +458,10 @@ def main():
     if not args.no_c3:
         line["c3_affine_all"] = c3_affine_all()
         progress(f"c3 affine_all: {line['c3_affine_all']['boxes_per_s']:.4g} boxes/s")
+    if not args.no_next:
+        line["next_rows"] = next_rows()
+        progress(f"next rows: frustum {line['next_rows']['frustum_1024']['rays_per_s']:.4g} rays/s, "
+                 f"sampling {line['next_rows']['sample_surface']['samples_per_s']:.4g} samples/s")
     if not args.no_concurrent:
         line["concurrent_trees"] = concurrent_trees()
         progress(f"concurrent trees: {line['concurrent_trees']['boxes_per_s_served']:.4g} boxes/s served")
