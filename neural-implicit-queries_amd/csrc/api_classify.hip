@@ -180,7 +180,10 @@ niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, in
   if (n > INT_MAX) return fail(NIQ_ERR_UNSUPPORTED, "FP64 re-test supports at most 2^31-1 boxes per call");
   void* scratch = nullptr;
   HIPCHK(hipMallocAsync(&scratch, sizeof(int) * (size_t)(n + 64), s));
-  HIPCHK(hipMemsetAsync(scratch, 0, sizeof(int), s));
+  if (const hipError_t e = hipMemsetAsync(scratch, 0, sizeof(int), s); e != hipSuccess) {
+    (void)hipFreeAsync(scratch, s);
+    return fail(NIQ_ERR_HIP, std::string("re-test counter clear: ") + hipGetErrorString(e));
+  }
   a.recheck_tol = tol;
   a.recheck_cnt = static_cast<int*>(scratch);
   a.recheck_idx = static_cast<int*>(scratch) + 64;

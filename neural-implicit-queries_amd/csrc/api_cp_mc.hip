@@ -208,12 +208,15 @@ extern "C" niq_status niq_mc_extract(const niq_model* m, const float* leaf_lower
   const int sd = n_subcell_depth;
   const long long np3 = (long long)((1 << sd) + 1) * ((1 << sd) + 1) * ((1 << sd) + 1);
   const long long chunk = std::max<long long>(1, std::min<long long>(n_leaves, (64LL << 20) / np3));
-  void *ppts, *pvals, *pcnt, *poff, *ptot;
-  HIPCHK(hipMallocAsync(&ppts, sizeof(float) * 3 * np3 * chunk, s));
-  HIPCHK(hipMallocAsync(&pvals, sizeof(float) * np3 * chunk, s));
-  HIPCHK(hipMallocAsync(&pcnt, sizeof(long long) * chunk, s));
-  HIPCHK(hipMallocAsync(&poff, sizeof(long long) * chunk, s));
-  HIPCHK(hipMallocAsync(&ptot, sizeof(long long), s));
+  void *ppts = nullptr, *pvals = nullptr, *pcnt = nullptr, *poff = nullptr, *ptot = nullptr;
+  const std::pair<void**, size_t> want[] = {{&ppts, sizeof(float) * 3 * np3 * chunk},
+                                            {&pvals, sizeof(float) * np3 * chunk},
+                                            {&pcnt, sizeof(long long) * chunk},
+                                            {&poff, sizeof(long long) * chunk},
+                                            {&ptot, sizeof(long long)}};
+  for (const auto& w : want)
+    if (st == NIQ_OK && hipMallocAsync(w.first, w.second, s) != hipSuccess)
+      st = fail(NIQ_ERR_HIP, "mc extract: scratch allocation failed");
   long long written = 0;
   for (long long l0 = 0; l0 < n_leaves && st == NIQ_OK; l0 += chunk) {
     const long long nl = std::min(chunk, n_leaves - l0);
@@ -235,7 +238,8 @@ extern "C" niq_status niq_mc_extract(const niq_model* m, const float* leaf_lower
     }
     written = tot;
   }
-  for (void* p : {ppts, pvals, pcnt, poff, ptot}) (void)hipFreeAsync(p, s);
+  for (void* p : {ppts, pvals, pcnt, poff, ptot})
+    if (p) (void)hipFreeAsync(p, s);
   if (st) return st;
   HIPCHK(hipGetLastError());
   *n_tris = written;
