@@ -109,6 +109,30 @@ def precision_compare(depth, ray_res=512):
                      "tflops": models.flops_per_box(dims, 5) * st["n_classified"] / dt / 1e12}
     res["speedup_bf16_over_fp32"] = res["fp32"]["seconds"] / res["bf16"]["seconds"]
     if ray_res > 0:
+        # the 2048^2 cast of this net in FP32, sampled: every 8th pixel of the
+        # 60 deg camera at distance 3 in both directions (256^2 rays), t in [0, 6]
+        f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+        eye = np.array([0., 0., 3.], np.float32)
+        look, upd, _ = render.look_at(eye)
+        roots, dirs = render.generate_camera_rays(eye, look, upd, res=2048, fov_deg=60.)
+        sel = torch.arange(0, 2048, 8)
+        pix = (sel[:, None] * 2048 + sel[None, :]).reshape(-1).to(roots.device)
+        roots, dirs = roots[pix].contiguous(), dirs[pix].contiguous()
+        opts = queries.get_default_cast_opts()
+        opts["max_dist"] = 6.0
+        queries.cast_rays((f,), (p,), roots[:1024], dirs[:1024], opts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hid, cnt, _ = queries.cast_rays((f,), (p,), roots, dirs, opts)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        steps = int(cnt.sum().item())
+        fl = models.flops_per_box(dims, rows=5)
+        res["rays_fp32"] = {"sample": "every 8th pixel of the 2048x2048 camera in both directions (65,536 rays)",
+                            "rays_per_s": roots.shape[0] / dt, "ray_steps_per_s": steps / dt, "seconds": dt,
+                            "hits": int((hid > 0).sum().item()), "flops_per_step": fl,
+                            "tflops": steps * fl / dt / 1e12,
+                            "frac_fp32_mfma_peak": steps * fl / dt / 1e12 / PEAK_FP32_TFLOPS}
         # cast_rays in both forms.  On this He-init net the precision mode's
         # a-priori widening exceeds the function's scale, so it certifies no
         # step; rays are timed on the reference's trained fox SDF instead
