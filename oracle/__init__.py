@@ -27,6 +27,22 @@ How each module is pinned (no reference golden vectors exist for this path):
   * ``rays``     - closed-form plane hits; no sign change before the hit.
   * ``closest``  - plane closed form (up to the reference's node-centre rule).
   * ``mc``       - table invariants; closed-form area of a plane cut.
+  * ``frustum``  - single-pixel frusta equal the single-ray oracle; a plane's
+                   hits; the subdivision partitions the image.
+  * ``intersect``- half-spaces (overlapping, disjoint, touching); grid brute
+                   force.
+  * ``sampling`` - Philox4x32-10 against the Random123 known-answer vectors;
+                   round-size independence against a one-at-a-time loop;
+                   closed-form slab, volume and centroid of half-spaces.
+  * ``funcs``    - the sdf mode against the exact ball-misses-plane test of a
+                   unit-normal plane; sign-constancy of determinate boxes.
+  * ``render``   - plane normals exactly; float64 gradients of a smooth net;
+                   the matcap lookup against scipy.ndimage.map_coordinates.
+The golden fixtures under tests/golden/ (written by make_golden.py) freeze the
+outputs of these modules for one seeded model, so later changes are caught.
 The TanH rule has no reference counterpart (the reference implements relu,
 elu and sin only); it is pinned by sampled soundness and FP64 invariants only.
+Parity unpinned (no available check fixes them): the FP32 bits of XLA's dot
+and reduction order, XLA's transcendental ulps, and the tie order of argsort /
+top_k in affine_truncate / affine_append (ties are broken lowest index first).
 """
