@@ -1,5 +1,6 @@
 """GPU parity of the ray and search queries across range-analysis modes and
-activations: cast_rays, cast_rays_frustum and closest_point.
+activations: cast_rays, cast_rays_frustum, closest_point and
+find_any_intersection.
 
 The ray caster bounds each step's segment with the mode's rule (interval,
 affine_fixed, slope_interval) and tests the two hit points with a plain
@@ -107,3 +108,24 @@ def test_closest_point_matches_oracle_per_mode(mode):
     same = d == rd
     note(f"  {int(same.sum())} of {same.size} distances identical; largest difference {np.abs(d - rd).max():.3g}")
     assert same.mean() >= 0.95 and np.abs(d - rd).max() <= 5 * 0.02
+
+
+@pytest.mark.parametrize("mode", ["interval", "slope_interval"])
+@pytest.mark.parametrize("seeds", [(11, 12), (13, 14)])
+def test_find_any_intersection_matches_oracle_per_mode(seeds, mode):
+    """find_any_intersection (src/kd_tree.py:402-655) with both functions in
+    another mode: the same found flag and point as the oracle."""
+    from oracle import intersect as oisect
+    from niq import models
+    note(f"[intersection per mode] seeds {seeds} {mode}")
+    a = models.random_mlp([32, 32], seed=seeds[0])
+    b = models.random_mlp([48, 24], activation="tanh", seed=seeds[1])
+    fa, _ = implicit_mlp_utils.implicit_from_params(a, mode)
+    fb, _ = implicit_mlp_utils.implicit_from_params(b, mode)
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    found, _, _, loc = kd_tree.find_any_intersection((fa, fb), (a, b), lo, up, 0.02)
+    rf, _, _, rloc, rn = oisect.find_any_intersection(ofuncs.OracleFunc(a, mode), ofuncs.OracleFunc(b, mode),
+                                                      lo, up, 0.02)
+    note(f"  gpu {found} {loc.tolist()}; oracle {rf} {rloc.tolist()} ({rn} nodes)")
+    assert found == rf
+    assert np.array_equal(loc.numpy(), rloc)
