@@ -226,6 +226,8 @@ def serving_trace(n_req, rate, seed=0):
                 reqs.append(srv.submit_closest_point(f, p, lo, up, inp, eps=0.05))
             else:
                 reqs.append(srv.submit_tree(f, p, lo, up, split_depth=arg))
+            if len(reqs) % 50 == 0:
+                progress(f"serving: {len(reqs)}/{len(plan)} requests submitted")
         for i, r in enumerate(reqs):
             r.result()
             if (i + 1) % 20 == 0:
