@@ -6,8 +6,9 @@ another (a merged cast gives every ray what a lone cast gives it), and each
 tree build and closest-point batch runs as one call of its own.  Checked with
 requests merged (one worker kept busy so that ray requests queue), with three
 workers overlapping on the device, with tree builds replayed from a captured
-hipGraph over different root boxes, and with memory admission serialising
-requests that do not fit together.
+hipGraph over different root boxes, with memory admission serialising
+requests that do not fit together, and with requests that cannot be replayed
+from a graph (affine_all, slope_interval at width 512) or use the sdf mode.
 """
 import sys
 import time
@@ -150,3 +151,39 @@ def test_memory_admission_serialises():
         with pytest.raises(_lib.CapacityError):
             srv.submit_tree(f1, p1, LO, UP, split_depth=18)
     assert same_tree(alone, ra) and same_tree(alone, rb)
+
+
+def test_other_modes_equal_lone_calls():
+    """Requests the serving loop cannot replay from a captured graph (growing
+    error-term rows, slope_interval planes in global scratch at width 512)
+    and the sdf mode: each result equals the standalone call's."""
+    import os
+    from conftest import GOLDEN
+    from niq import mlp as nmlp
+    note("[serve] other modes")
+    p_tanh = models.random_mlp([48, 40, 24], activation="tanh", seed=3)
+    f_all, _ = implicit_mlp_utils.implicit_from_params(p_tanh, "affine_all")
+    p_wide = models.random_mlp([512, 512], seed=4)
+    f_slope, _ = implicit_mlp_utils.implicit_from_params(p_wide, "slope_interval")
+    p_fox = nmlp.load(os.path.join(GOLDEN, "fox.npz"))
+    f_sdf, _ = implicit_mlp_utils.implicit_from_params(p_fox, "sdf")
+    f_c1, _ = implicit_mlp_utils.implicit_from_params(models.config_model("c1_relu_3x32"), "slope_interval")
+    p_c1 = models.config_model("c1_relu_3x32")
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    roots, dirs = camera(32)
+    q = torch.as_tensor(np.random.default_rng(2).uniform(-1, 1, size=(40, 3)).astype(np.float32), device="cuda")
+    alone_all = kd_tree.construct_uniform_unknown_levelset_tree(f_all, p_tanh, LO, UP, split_depth=9)
+    alone_slope = kd_tree.construct_uniform_unknown_levelset_tree(f_slope, p_wide, LO, UP, split_depth=6)
+    alone_rays = queries.cast_rays((f_sdf,), (p_fox,), roots, dirs, opts)
+    alone_cp = kd_tree.closest_point(f_c1, p_c1, LO, UP, q, eps=0.05)
+    with serve.Server(n_streams=3) as srv:
+        a = srv.submit_tree(f_all, p_tanh, LO, UP, split_depth=9)
+        b = srv.submit_tree(f_slope, p_wide, LO, UP, split_depth=6)
+        c = srv.submit_rays((f_sdf,), (p_fox,), roots, dirs, opts)
+        d = srv.submit_closest_point(f_c1, p_c1, LO, UP, q, eps=0.05)
+        ra, rb, rc, rd = a.result(), b.result(), c.result(), d.result()
+        assert not a.info["graph_replay"] and not b.info["graph_replay"]
+    assert same_tree(alone_all, ra) and same_tree(alone_slope, rb)
+    assert same_rays(alone_rays, rc)
+    assert torch.equal(rd[0], alone_cp[0]) and torch.equal(rd[1], alone_cp[1])
