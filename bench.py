@@ -108,31 +108,42 @@ def precision_compare(depth, ray_res=512):
                      "unknown_leaves": st["n_unknown"],
                      "tflops": models.flops_per_box(dims, 5) * st["n_classified"] / dt / 1e12}
     res["speedup_bf16_over_fp32"] = res["fp32"]["seconds"] / res["bf16"]["seconds"]
+    res["bound_quality"] = {name: bf16_bound_quality(name) for name in ("c4_relu_512x8", "fox", "bunny")}
+    progress("precision: bound quality " + ", ".join(
+        f"{k} median inflation vs fp64 {v['inflation_vs_fp64']['median']:.3f}, false-empty {v['false_empty']}"
+        for k, v in res["bound_quality"].items()))
     if ray_res > 0:
-        # the 2048^2 cast of this net in FP32, sampled: every 8th pixel of the
-        # 60 deg camera at distance 3 in both directions (256^2 rays), t in [0, 6]
-        f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+        # the 2048^2 cast of this net (query B), sampled: every 4th pixel of the
+        # 60 deg camera at distance 3 in both directions (512^2 rays, 1/16 of
+        # the image), t in [0, 6], in FP32 and in the precision mode
         eye = np.array([0., 0., 3.], np.float32)
         look, upd, _ = render.look_at(eye)
         roots, dirs = render.generate_camera_rays(eye, look, upd, res=2048, fov_deg=60.)
-        sel = torch.arange(0, 2048, 8)
+        sel = torch.arange(0, 2048, 4)
         pix = (sel[:, None] * 2048 + sel[None, :]).reshape(-1).to(roots.device)
         roots, dirs = roots[pix].contiguous(), dirs[pix].contiguous()
         opts = queries.get_default_cast_opts()
         opts["max_dist"] = 6.0
-        queries.cast_rays((f,), (p,), roots[:1024], dirs[:1024], opts)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, hid, cnt, _ = queries.cast_rays((f,), (p,), roots, dirs, opts)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        steps = int(cnt.sum().item())
         fl = models.flops_per_box(dims, rows=5)
-        res["rays_fp32"] = {"sample": "every 8th pixel of the 2048x2048 camera in both directions (65,536 rays)",
-                            "rays_per_s": roots.shape[0] / dt, "ray_steps_per_s": steps / dt, "seconds": dt,
-                            "hits": int((hid > 0).sum().item()), "flops_per_step": fl,
-                            "tflops": steps * fl / dt / 1e12,
-                            "frac_fp32_mfma_peak": steps * fl / dt / 1e12 / PEAK_FP32_TFLOPS}
+        for prec in ("fp32", "bf16"):
+            f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision=prec)
+            queries.cast_rays((f,), (p,), roots[:1024], dirs[:1024], opts)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, hid, cnt, _ = queries.cast_rays((f,), (p,), roots, dirs, opts)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            steps = int(cnt.sum().item())
+            line = {"sample": "every 4th pixel of the 2048x2048 camera in both directions (262,144 rays, 1/16)",
+                    "rays_per_s": roots.shape[0] / dt, "ray_steps_per_s": steps / dt, "seconds": dt,
+                    "hits": int((hid > 0).sum().item()), "ray_steps": steps, "flops_per_step": fl,
+                    "full_image_seconds_estimate": dt * 16}
+            if prec == "fp32":
+                line["tflops"] = steps * fl / dt / 1e12
+                line["frac_fp32_mfma_peak"] = line["tflops"] / PEAK_FP32_TFLOPS
+            res[f"rays_{prec}"] = line
+            progress(f"precision: 3->512x8->1 rays {prec}: {roots.shape[0] / dt:.4g} rays/s, "
+                     f"{int((hid > 0).sum().item())} hits")
         # cast_rays in both forms.  On this He-init net the precision mode's
         # a-priori widening exceeds the function's scale, so it certifies no
         # step; rays are timed on the reference's trained fox SDF instead
@@ -157,6 +168,36 @@ def precision_compare(depth, ray_res=512):
         rays["speedup_bf16_over_fp32"] = rays["fp32"]["seconds"] / rays["bf16"]["seconds"]
         res["rays"] = rays
     return res
+
+
+def bf16_bound_quality(name, n_cells=512, seed=4):
+    """Precision-mode bounds at the finest cells of a depth-21 tree (half-width
+    1/128, random centres): width inflation against the plain FP32 kernel and
+    against the FP64 oracle (median, p90, max), and the false-empty count (a
+    cell the precision mode calls POSITIVE or NEGATIVE whose FP64 bound does
+    not), with the number of cells either form calls determinate."""
+    import torch
+    from niq import models, implicit_mlp_utils, mlp as nmlp
+    from oracle import funcs as ofuncs, affine as oaff
+    p = (models.config_model(name) if name in models.CONFIGS
+         else nmlp.load(os.path.join(ROOT, "tests", "golden", f"{name}.npz")))
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, size=(n_cells, 3))
+    lo, up = (c - 1 / 128).astype(np.float32), (c + 1 / 128).astype(np.float32)
+    tl, tu = torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda")
+    f16, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision="bf16")
+    f32, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    c16, b16 = (x.cpu().numpy() for x in f16.classify_box(p, tl, tu, return_bounds=True))
+    c32, b32 = (x.cpu().numpy() for x in f32.classify_box(p, tl, tu, return_bounds=True))
+    d_lo, d_hi = ofuncs.OracleFunc(p, "affine_fixed", np.float64).bounds_box(lo, up)
+    c64 = oaff.sign_code(d_lo, d_hi, 0.0)
+    w16 = b16[:, 1] - b16[:, 0]
+    stats = lambda r: {"median": float(np.median(r)), "p90": float(np.percentile(r, 90)), "max": float(np.max(r))}
+    return {"cells": n_cells, "half_width": 1 / 128,
+            "inflation_vs_fp32": stats(w16 / (b32[:, 1] - b32[:, 0])),
+            "inflation_vs_fp64": stats(w16 / (d_hi - d_lo)),
+            "false_empty": int(((c16 != 0) & (c16 != c64)).sum()),
+            "determinate_bf16": int((c16 != 0).sum()), "determinate_fp64": int((c64 != 0).sum())}
 
 
 SERVE_SHAPES = (("c1 3->32x3->1 relu", [32] * 3, "relu", "he"),
@@ -313,9 +354,21 @@ def c3_affine_all(depth=21):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     lv = st["level_counts"]
+    dims = [3] + [128] * 8 + [1]
+    fl = models.flops_per_box_affine_all(dims)
+    eb = models.error_bytes_per_box_affine_all(dims)
+    tflops = fl * st["n_classified"] / dt / 1e12
+    gbs = eb * st["n_classified"] / dt / 1e9
     return {"workload": f"levelset tree, TanH MLP 3->128x8->1, affine_all, split depth {depth}",
             "seconds": dt, "boxes": st["n_classified"], "boxes_per_s": st["n_classified"] / dt,
-            "unknown_leaves": st["n_unknown"], "active_boxes_per_level": [min(lv), max(lv)]}
+            "unknown_leaves": st["n_unknown"], "active_boxes_per_level": [min(lv), max(lv)],
+            # the whole tree's algorithmic work over its wall time: the
+            # error-term contraction (FP32 MFMA) and the error-term matrix
+            # streamed through HBM (read and written once per layer)
+            "roofline": {"kernel": "k_classify_rows (affine_all, W=128)", "flops_per_box": fl,
+                         "achieved_tflops": tflops, "peak_tflops": PEAK_FP32_TFLOPS,
+                         "frac_mfma": tflops / PEAK_FP32_TFLOPS, "hbm_bytes_per_box": eb,
+                         "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS, "frac_hbm": gbs / PEAK_HBM_GBS}}
 
 
 def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):

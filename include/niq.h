@@ -69,7 +69,8 @@ typedef struct niq_eval_cfg {
                           the packed bf16 weight copies, 2 splits FP32 weights
                           in registers (the default) */
   float sdf_lipschitz; /* NIQ_MODE_SDF: Lipschitz bound of f (sdf_lipschitz kwarg) */
-  int reserved;
+  int ray_variant;     /* cast_rays tile shape, 0 = default; 1 and 2 are the
+                          alternatives the autotuner times (same results) */
 } niq_eval_cfg;
 
 /* ---- model ------------------------------------------------------------
@@ -339,16 +340,23 @@ niq_status niq_shade_image(const niq_shade_opts* opts, const float* ray_dirs, co
  * n_streams worker threads, each with its own HIP stream at the default
  * priority.  Requests queue in arrival order; a worker takes the oldest one as
  * soon as its device memory fits under mem_cap_bytes beside the work in
- * flight.  Ray requests waiting in the queue with the same models, eval
- * config and options are merged into the cast of the request taken (up to
- * max_merge_rays rays): every ray is independent, so merged results equal
- * lone ones bit for bit.  Tree builds in the fused-kernel modes with a
+ * flight.  Ray requests waiting in the queue with the same eval config and
+ * template width are merged into the cast of the request taken (up to 64
+ * requests and max_merge_rays rays): each request is a group of rays with its
+ * own models, options, inputs and outputs inside one persistent kernel, and
+ * every ray is independent, so merged results (N_evals included) equal lone
+ * ones bit for bit.  Tree builds in the fused-kernel modes with a
  * split_depth replay a hipGraph captured once per (model, config, depth,
  * capacities) on each worker.  Closest-point batches run as they are.
  *
  * Buffers: inputs must be complete on the device when submitted and stay
  * alive, and outputs untouched, until the request has been polled done.
- * Model handles must outlive the requests that use them. */
+ * Model handles must outlive the requests that use them.  A cached graph is
+ * matched to its model by a serial number unique to each niq_model_create,
+ * never by handle address, so a model created after another was destroyed
+ * never replays the old model's graph (such graphs are evicted unused).
+ * Polling one id from several threads is safe: one poll returns the result,
+ * the others NIQ_ERR_INVALID. */
 typedef struct niq_server niq_server;
 
 typedef struct niq_serve_info {
@@ -356,9 +364,11 @@ typedef struct niq_serve_info {
   niq_status status;     /* the request's own status once done */
   double queue_ms;       /* submit -> start */
   double latency_ms;     /* submit -> done */
-  int64_t n_evals;       /* rays: N_evals of the cast that served the request */
-  int32_t n_batched;     /* rays: requests merged into that cast */
-  int32_t graph_replay;  /* trees: 1 if served by replaying a captured graph */
+  int64_t n_evals;       /* rays: the request's N_evals, as a lone cast gives it */
+  int32_t n_batched;     /* rays: requests merged into the cast that served it */
+  int32_t graph_replay;  /* trees: 0 built without a graph; 1 served by a graph
+                            captured for this request; 2 by replaying a graph
+                            cached from an earlier request */
   int64_t n_processed;   /* closest point: nodes processed */
 } niq_serve_info;
 

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "niq", "libniq.so")
-SOURCES = ["api_model.hip", "api_classify.hip", "api_classify_bf16.hip", "api_classify_rows.hip", "api_tree.hip", "api_rays.hip", "api_cp_mc.hip",
+SOURCES = ["api_model.hip", "api_classify.hip", "api_classify_bf16.hip", "api_classify_rows.hip", "api_tree.hip", "api_rays.hip", "api_rays_rows.hip", "api_cp_mc.hip",
            "api_serve.hip", "api_frustum.hip",
            "api_isect.hip", "api_sample.hip", "api_render.hip"]
 

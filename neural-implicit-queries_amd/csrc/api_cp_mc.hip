@@ -24,25 +24,53 @@ extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* 
   hipStream_t s = (hipStream_t)stream;
   const int K = 8;                                  // rounds between host checks
   long long cap = std::max<long long>(2 * Q, Q + 2LL * K * B);
-  // per-call device scratch (stream ordered)
-  auto al = [&](size_t bytes, void** p) { return hipMallocAsync(p, (bytes + 255) / 256 * 256, s); };
-  void *pS, *pbq, *pblo, *pbup, *ppts, *pcodes, *pvals, *pwin, *pidist, *pineeds, *pwq, *pwlo, *pwup;
-  HIPCHK(al(sizeof(CpState), &pS));
-  HIPCHK(al(sizeof(int) * B, &pbq));
-  HIPCHK(al(sizeof(float) * 3 * B, &pblo));
-  HIPCHK(al(sizeof(float) * 3 * B, &pbup));
-  HIPCHK(al(sizeof(float) * 21 * B, &ppts));
-  HIPCHK(al(sizeof(int) * B, &pcodes));
-  HIPCHK(al(sizeof(float) * 7 * B, &pvals));
-  HIPCHK(al(sizeof(int) * Q, &pwin));
-  HIPCHK(al(sizeof(float) * B, &pidist));
-  HIPCHK(al(sizeof(int) * B, &pineeds));
-  HIPCHK(al(sizeof(int) * cap, &pwq));
-  HIPCHK(al(sizeof(float) * 3 * cap, &pwlo));
-  HIPCHK(al(sizeof(float) * 3 * cap, &pwup));
+  // per-call device scratch (stream ordered); every allocation made is freed
+  // on every path, the stack arrays (which may be replaced as they grow) at the
+  // end through `a`
+  niq_status st = NIQ_OK;
+  std::vector<void*> owned;
+  auto al = [&](size_t bytes, void** p) {
+    *p = nullptr;
+    const hipError_t e = hipMallocAsync(p, (bytes + 255) / 256 * 256, s);
+    if (e != hipSuccess) *p = nullptr;
+    return e;
+  };
+  auto own = [&](size_t bytes, void** p) {
+    if (st != NIQ_OK) { *p = nullptr; return; }
+    const hipError_t e = al(bytes, p);
+    if (e != hipSuccess) st = fail(NIQ_ERR_HIP, std::string("closest_point: hipMallocAsync: ") + hipGetErrorString(e));
+    else owned.push_back(*p);
+  };
+  auto hipfail = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && st == NIQ_OK) st = fail(NIQ_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  void *pS, *pbq, *pblo, *pbup, *ppts, *pcodes, *pvals, *pwin, *pidist, *pineeds;
+  own(sizeof(CpState), &pS);
+  own(sizeof(int) * B, &pbq);
+  own(sizeof(float) * 3 * B, &pblo);
+  own(sizeof(float) * 3 * B, &pbup);
+  own(sizeof(float) * 21 * B, &ppts);
+  own(sizeof(int) * B, &pcodes);
+  own(sizeof(float) * 7 * B, &pvals);
+  own(sizeof(int) * Q, &pwin);
+  own(sizeof(float) * B, &pidist);
+  own(sizeof(int) * B, &pineeds);
   CpArgs a{};
-  a.B = B; a.cap = cap;
+  void *pwq = nullptr, *pwlo = nullptr, *pwup = nullptr;
+  if (st == NIQ_OK) hipfail(al(sizeof(int) * cap, &pwq), "closest_point: hipMallocAsync");
+  if (st == NIQ_OK) hipfail(al(sizeof(float) * 3 * cap, &pwlo), "closest_point: hipMallocAsync");
+  if (st == NIQ_OK) hipfail(al(sizeof(float) * 3 * cap, &pwup), "closest_point: hipMallocAsync");
   a.wq = (int*)pwq; a.wlo = (float*)pwlo; a.wup = (float*)pwup;
+  auto release = [&]() {
+    for (void* p : owned) (void)hipFreeAsync(p, s);
+    for (void* p : {(void*)a.wq, (void*)a.wlo, (void*)a.wup})
+      if (p) (void)hipFreeAsync(p, s);
+  };
+  if (st != NIQ_OK) {
+    release();
+    return st;
+  }
+  a.B = B; a.cap = cap;
   a.bq = (int*)pbq; a.blo = (float*)pblo; a.bup = (float*)pbup; a.pts = (float*)ppts;
   a.codes = (int*)pcodes; a.vals = (float*)pvals; a.q = query_points;
   a.qmin = out_dist; a.qloc = out_loc; a.win = (int*)pwin;
@@ -53,24 +81,29 @@ extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* 
   {
     CpState h{};
     h.top = (int)Q;
-    HIPCHK(hipMemcpyAsync(S, &h, sizeof(h), hipMemcpyHostToDevice, s));
     std::vector<int> ids(Q);
     for (long long i = 0; i < Q; ++i) ids[i] = (int)i;
     std::vector<float> blo(3 * Q), bup(3 * Q), inf(Q, INFINITY), loc(3 * Q, -777.f);
     for (long long i = 0; i < Q; ++i)
       for (int k = 0; k < 3; ++k) { blo[3 * i + k] = lower[k]; bup[3 * i + k] = upper[k]; }
-    HIPCHK(hipMemcpyAsync(a.wq, ids.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(a.wlo, blo.data(), sizeof(float) * 3 * Q, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(a.wup, bup.data(), sizeof(float) * 3 * Q, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(out_dist, inf.data(), sizeof(float) * Q, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(out_loc, loc.data(), sizeof(float) * 3 * Q, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemsetAsync(a.win, 0xff, sizeof(int) * Q, s));
-    HIPCHK(hipStreamSynchronize(s));   // host vectors go out of scope
+    hipfail(hipMemcpyAsync(S, &h, sizeof(h), hipMemcpyHostToDevice, s), "closest_point: set-up copy");
+    if (st == NIQ_OK) hipfail(hipMemcpyAsync(a.wq, ids.data(), sizeof(int) * Q, hipMemcpyHostToDevice, s), "closest_point: set-up copy");
+    if (st == NIQ_OK) hipfail(hipMemcpyAsync(a.wlo, blo.data(), sizeof(float) * 3 * Q, hipMemcpyHostToDevice, s), "closest_point: set-up copy");
+    if (st == NIQ_OK) hipfail(hipMemcpyAsync(a.wup, bup.data(), sizeof(float) * 3 * Q, hipMemcpyHostToDevice, s), "closest_point: set-up copy");
+    if (st == NIQ_OK) hipfail(hipMemcpyAsync(out_dist, inf.data(), sizeof(float) * Q, hipMemcpyHostToDevice, s), "closest_point: set-up copy");
+    if (st == NIQ_OK) hipfail(hipMemcpyAsync(out_loc, loc.data(), sizeof(float) * 3 * Q, hipMemcpyHostToDevice, s), "closest_point: set-up copy");
+    if (st == NIQ_OK) hipfail(hipMemsetAsync(a.win, 0xff, sizeof(int) * Q, s), "closest_point: set-up memset");
+    // the host vectors and h go out of scope here: wait for their copies on
+    // every path
+    hipfail(hipStreamSynchronize(s), "closest_point: hipStreamSynchronize");
+  }
+  if (st != NIQ_OK) {
+    release();
+    return st;
   }
   ClassifyArgs ca{};
   ca.lower = a.blo; ca.upper = a.bup; ca.d = 3; ca.n_dev = &S->n; ca.offset = 0.f; ca.codes = a.codes;
   const unsigned gg = (unsigned)std::min<long long>((B + 255) / 256, 1024);
-  niq_status st = NIQ_OK;
   while (true) {
     for (int r = 0; r < K && st == NIQ_OK; ++r) {
       hipLaunchKernelGGL(k_cp_gather, dim3(gg), dim3(256), 0, s, S, a);
@@ -98,8 +131,10 @@ extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* 
     }
     if (h.top + 2LL * K * B > cap) {   // grow the stack before the next rounds
       const long long ncap = std::max(2 * cap, (long long)h.top + 4LL * K * B);
-      void *nq, *nlo, *nup;
+      void *nq = nullptr, *nlo = nullptr, *nup = nullptr;
       if (al(sizeof(int) * ncap, &nq) || al(sizeof(float) * 3 * ncap, &nlo) || al(sizeof(float) * 3 * ncap, &nup)) {
+        for (void* p : {nq, nlo, nup})
+          if (p) (void)hipFreeAsync(p, s);
         st = fail(NIQ_ERR_HIP, "closest_point: stack growth failed");
         break;
       }
@@ -110,8 +145,7 @@ extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* 
       a.wq = (int*)nq; a.wlo = (float*)nlo; a.wup = (float*)nup; a.cap = ncap; cap = ncap;
     }
   }
-  for (void* p : {pS, pbq, pblo, pbup, ppts, pcodes, pvals, pwin, pidist, pineeds}) (void)hipFreeAsync(p, s);
-  (void)hipFreeAsync(a.wq, s); (void)hipFreeAsync(a.wlo, s); (void)hipFreeAsync(a.wup, s);
+  release();
   if (st == NIQ_OK) HIPCHK(hipGetLastError());
   return st;
 }

@@ -27,8 +27,9 @@ extern "C" niq_status niq_cast_rays_frustum(const niq_model* const* models, int 
   for (int i = 0; i < n_models; ++i)
     if (niq_status st = check_model(models[i], 3)) return st;
   const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
-  if (mode == NIQ_MODE_AFFINE_ALL || mode == NIQ_MODE_AFFINE_TRUNCATE || mode == NIQ_MODE_AFFINE_APPEND)
-    return fail(NIQ_ERR_UNSUPPORTED, "frustum casting runs the fused-kernel modes");
+  if (cfg && cfg->precision == NIQ_PREC_BF16 &&
+      (mode == NIQ_MODE_AFFINE_ALL || mode == NIQ_MODE_AFFINE_TRUNCATE || mode == NIQ_MODE_AFFINE_APPEND))
+    return fail(NIQ_ERR_UNSUPPORTED, "the precision mode runs interval and affine_fixed");
   hipStream_t s = (hipStream_t)stream;
   const long long cap = (long long)rx * ry;   // frusta partition the image
 

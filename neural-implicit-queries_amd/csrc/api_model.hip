@@ -1,6 +1,8 @@
 // Model handles: lowering upload, FP32 fragments and the precision-mode copy.
 #include "niq_internal.h"
 
+#include <atomic>
+
 namespace niqi {
 thread_local std::string g_err;
 }  // namespace niqi
@@ -64,7 +66,9 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
     for (int f = 0; f < outd; ++f) bp[f] = (b && b[i]) ? b[i][f] : 0.f;
   }
 
+  static std::atomic<uint64_t> next_serial{1};
   niq_model* m = new niq_model();
+  m->serial = next_serial.fetch_add(1);
   m->n_layers = n_layers;
   m->in_dim = dims[0];
   m->wt = wt;

@@ -1,5 +1,6 @@
 // Serving loop: worker threads with one HIP stream each, admission by device
-// memory, merged ray casts and replayed tree-build graphs.
+// memory, ray requests merged into one persistent cast (one ray group each)
+// and replayed tree-build graphs.
 #include "niq_internal.h"
 
 #include <chrono>
@@ -58,16 +59,31 @@ struct Request {
   int64_t n_processed = 0;
 };
 
+// Template widths of a ray request's functions (FP32 and precision-mode
+// copies): requests merge into one cast only at equal widths, so no function
+// runs in a wider tile than its own cast would give it.
+int req_wt(const Request& r, bool w16) {
+  int w = 0;
+  for (const niq_model* m : r.models) w = std::max(w, w16 ? m->wt16 : m->wt);
+  return w;
+}
+
+// Ray requests that can share one persistent cast: the same eval config
+// (mode, precision, variants) and template widths.  Models, options and
+// sizes may differ: each request is a group of the cast with its own
+// functions, options, inputs and outputs.
 bool same_cast(const Request& a, const Request& b) {
-  return a.kind == REQ_RAYS && b.kind == REQ_RAYS && a.models == b.models &&
-         std::memcmp(&a.cfg, &b.cfg, sizeof(niq_eval_cfg)) == 0 &&
-         std::memcmp(&a.opts, &b.opts, sizeof(niq_cast_opts)) == 0;
+  return a.kind == REQ_RAYS && b.kind == REQ_RAYS && std::memcmp(&a.cfg, &b.cfg, sizeof(niq_eval_cfg)) == 0 &&
+         req_wt(a, false) == req_wt(b, false) && req_wt(a, true) == req_wt(b, true);
 }
 
 // One captured tree build: its workspace, internal outputs and root box stay
 // allocated while the graph is cached.
 struct TreeGraph {
-  const niq_model* model = nullptr;
+  // The model is matched by its serial number, not its address: a model
+  // destroyed and another created can share an address, and a graph's
+  // captured arguments hold the old model's device weights.
+  uint64_t model_serial = 0;
   niq_eval_cfg cfg{};
   int split_depth = 0, B = 0, with_int = 0, with_ext = 0;
   float offset = 0.f;
@@ -81,7 +97,7 @@ struct TreeGraph {
   hipGraphExec_t exec = nullptr;
 
   bool matches(const Request& r) const {
-    return model == r.model && std::memcmp(&cfg, &r.cfg, sizeof(cfg)) == 0 && split_depth == r.tp.split_depth &&
+    return model_serial == r.model->serial && std::memcmp(&cfg, &r.cfg, sizeof(cfg)) == 0 && split_depth == r.tp.split_depth &&
            B == r.tp.batch_process_size && with_int == r.tp.with_interior_nodes &&
            with_ext == r.tp.with_exterior_nodes && offset == r.tp.offset && cap_unk == r.tres.unknown_capacity &&
            cap_int == (r.tp.with_interior_nodes ? r.tres.interior_capacity : 0) &&
@@ -135,77 +151,55 @@ struct Worker {
   void run_rays(std::vector<Request*>& group);
   void run_tree(Request* r);
   void run_cp(Request* r);
-  TreeGraph* graph_for(Request* r);
+  TreeGraph* graph_for(Request* r, bool* reused);
   void clear_graphs();
 };
 
 void Worker::run_rays(std::vector<Request*>& group) {
-  Request* r0 = group[0];
-  const size_t ws_bytes = niq_cast_workspace_bytes(&r0->opts);
-  int64_t total = 0;
-  for (Request* r : group) total += r->n;
+  // one persistent cast over every merged request, each its own ray group
+  const int ng = (int)group.size();
+  std::vector<CastGroupReq> reqs(ng);
+  int wt = 0, wt16 = 0;
+  for (int g = 0; g < ng; ++g) {
+    Request* r = group[g];
+    CastGroupReq& q = reqs[g];
+    q.models = r->models.data(); q.n_models = (int)r->models.size();
+    q.roots = r->roots; q.dirs = r->dirs; q.n = r->n; q.opts = r->opts;
+    q.out_t = r->out_t; q.out_hit = r->out_hit; q.out_count = r->out_count;
+    wt = std::max(wt, req_wt(*r, false));
+    wt16 = std::max(wt16, req_wt(*r, true));
+  }
+  const niq_eval_cfg& cfg = group[0]->cfg;
+  const size_t ws_bytes = cast_layout(reqs.data(), ng, cfg.precision == NIQ_PREC_BF16).total;
   niq_status st = NIQ_OK;
   void* ws = nullptr;
   auto hipfail = [&](hipError_t e, const char* what) {
     if (e != hipSuccess && st == NIQ_OK) st = fail(NIQ_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   };
   hipfail(hipMallocAsync(&ws, ws_bytes, s), "hipMallocAsync");
-  int64_t n_evals = 0;
-  if (group.size() == 1) {
-    if (st == NIQ_OK)
-      st = niq_cast_rays(r0->models.data(), (int)r0->models.size(), &r0->cfg, r0->roots, r0->dirs, r0->n, &r0->opts,
-                         r0->out_t, r0->out_hit, r0->out_count, ws, ws_bytes, &n_evals, s);
-  } else {
-    // one cast over the concatenated rays; inputs gathered, outputs scattered
-    // back with stream-ordered copies
-    void* buf = nullptr;
-    const size_t bytes = (size_t)total * 36;
-    hipfail(hipMallocAsync(&buf, bytes, s), "hipMallocAsync");
-    float* roots = static_cast<float*>(buf);
-    float* dirs = roots + 3 * total;
-    float* t = dirs + 3 * total;
-    int32_t* hit = reinterpret_cast<int32_t*>(t + total);
-    int32_t* cnt = hit + total;
-    int64_t off = 0;
-    for (Request* r : group) {
-      if (st != NIQ_OK) break;
-      hipfail(hipMemcpyAsync(roots + 3 * off, r->roots, 12 * r->n, hipMemcpyDeviceToDevice, s), "gather");
-      hipfail(hipMemcpyAsync(dirs + 3 * off, r->dirs, 12 * r->n, hipMemcpyDeviceToDevice, s), "gather");
-      off += r->n;
-    }
-    if (st == NIQ_OK)
-      st = niq_cast_rays(r0->models.data(), (int)r0->models.size(), &r0->cfg, roots, dirs, total, &r0->opts, t, hit,
-                         cnt, ws, ws_bytes, &n_evals, s);
-    off = 0;
-    for (Request* r : group) {
-      if (st != NIQ_OK) break;
-      hipfail(hipMemcpyAsync(r->out_t, t + off, 4 * r->n, hipMemcpyDeviceToDevice, s), "scatter");
-      hipfail(hipMemcpyAsync(r->out_hit, hit + off, 4 * r->n, hipMemcpyDeviceToDevice, s), "scatter");
-      hipfail(hipMemcpyAsync(r->out_count, cnt + off, 4 * r->n, hipMemcpyDeviceToDevice, s), "scatter");
-      off += r->n;
-    }
-    if (buf) hipfail(hipFreeAsync(buf, s), "hipFreeAsync");
-  }
+  if (st == NIQ_OK) st = cast_run_groups(reqs.data(), ng, &cfg, static_cast<unsigned char*>(ws), wt, wt16, s);
   if (ws) hipfail(hipFreeAsync(ws, s), "hipFreeAsync");
   hipfail(hipStreamSynchronize(s), "hipStreamSynchronize");
-  for (Request* r : group) {
-    r->n_evals = n_evals;
-    r->n_batched = (int)group.size();
-    finish(r, st);
+  for (int g = 0; g < ng; ++g) {
+    group[g]->n_evals = reqs[g].n_evals;
+    group[g]->n_batched = ng;
+    finish(group[g], st);
   }
 }
 
-TreeGraph* Worker::graph_for(Request* r) {
+TreeGraph* Worker::graph_for(Request* r, bool* reused) {
+  *reused = false;
   for (auto it = graphs.begin(); it != graphs.end(); ++it) {
     if (it->matches(*r)) {
       graphs.splice(graphs.begin(), graphs, it);
+      *reused = true;
       return &graphs.front();
     }
   }
   size_t ws_bytes = 0;
   if (niq_tree_workspace_bytes(r->model, &r->cfg, &r->tp, &ws_bytes) != NIQ_OK) return nullptr;
   TreeGraph g;
-  g.model = r->model; g.cfg = r->cfg; g.split_depth = r->tp.split_depth; g.B = r->tp.batch_process_size;
+  g.model_serial = r->model->serial; g.cfg = r->cfg; g.split_depth = r->tp.split_depth; g.B = r->tp.batch_process_size;
   g.with_int = r->tp.with_interior_nodes; g.with_ext = r->tp.with_exterior_nodes; g.offset = r->tp.offset;
   g.cap_unk = r->tres.unknown_capacity;
   g.cap_int = g.with_int ? r->tres.interior_capacity : 0;
@@ -271,7 +265,8 @@ void Worker::run_tree(Request* r) {
   const bool capturable = r->tp.split_depth >= 0 && r->tp.node_terminate_thresh <= 0 &&
                           mode != NIQ_MODE_AFFINE_ALL && mode != NIQ_MODE_AFFINE_TRUNCATE &&
                           mode != NIQ_MODE_AFFINE_APPEND && !classify_needs_scratch(r->model, &r->cfg, 3);
-  TreeGraph* g = capturable ? graph_for(r) : nullptr;
+  bool reused = false;
+  TreeGraph* g = capturable ? graph_for(r, &reused) : nullptr;
   niq_status st = NIQ_OK;
   auto hipfail = [&](hipError_t e, const char* what) {
     if (e != hipSuccess && st == NIQ_OK) st = fail(NIQ_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -302,7 +297,7 @@ void Worker::run_tree(Request* r) {
              g->cap_ext, r->tres.n_exterior);
       hipfail(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
-    r->graph_replay = 1;
+    r->graph_replay = reused ? 2 : 1;
   } else {
     size_t ws_bytes = 0;
     void* ws = nullptr;
@@ -356,7 +351,7 @@ void niq_server::worker_main() {
         // continuous batching: fold in every waiting cast of the same models,
         // config and options that still fits
         int64_t rays = r->n;
-        for (auto it = queue.begin(); it != queue.end();) {
+        for (auto it = queue.begin(); it != queue.end() && (int)group.size() < kMaxRayGroups;) {
           Request* o = *it;
           if (same_cast(*r, *o) && rays + o->n <= max_merge && in_use + need + o->need <= cap) {
             group.push_back(o);
@@ -452,16 +447,16 @@ niq_status niq_serve_submit_rays(niq_server* srv, const niq_model* const* models
   if (!models || n_models < 1 || n_models > 8) return fail(NIQ_ERR_INVALID, "need 1..8 models");
   if (!opts) return fail(NIQ_ERR_INVALID, "null opts");
   if (n < 0 || n > srv->max_merge) return fail(NIQ_ERR_INVALID, "ray count out of range");
-  if (n > 0 && (!roots || !dirs || !out_t || !out_hit_id || !out_count)) return fail(NIQ_ERR_INVALID, "bad buffers");
-  for (int i = 0; i < n_models; ++i)
-    if (niq_status st = check_model(models[i], 3)) return st;
+  int wt = 0, wt16 = 0;
+  if (niq_status st = cast_check(models, n_models, cfg, roots, dirs, n, opts, out_t, out_hit_id, out_count, &wt, &wt16))
+    return st;
   auto r = std::make_unique<Request>();
   r->kind = REQ_RAYS;
   r->models.assign(models, models + n_models);
   r->cfg = cfg ? *cfg : default_cfg();
   r->roots = roots; r->dirs = dirs; r->n = n; r->opts = *opts;
   r->out_t = out_t; r->out_hit = out_hit_id; r->out_count = out_count;
-  r->need = niq_cast_workspace_bytes(opts) + 36 * (size_t)n + 256;
+  r->need = niq_cast_workspace_bytes(opts);
   return srv->submit(std::move(r), request_id);
 }
 
@@ -521,8 +516,18 @@ niq_status niq_serve_poll(niq_server* srv, uint64_t request_id, int wait, niq_se
     std::unique_lock<std::mutex> lk(srv->mu);
     auto it = srv->reqs.find(request_id);
     if (it == srv->reqs.end()) return fail(NIQ_ERR_INVALID, "unknown request id " + std::to_string(request_id));
+    if (wait) {
+      // look the id up again on every wake: another poller may have taken the
+      // request (and freed it) while this one waited
+      srv->cv_done.wait(lk, [&] {
+        auto f = srv->reqs.find(request_id);
+        return f == srv->reqs.end() || f->second->state == 2;
+      });
+      it = srv->reqs.find(request_id);
+      if (it == srv->reqs.end())
+        return fail(NIQ_ERR_INVALID, "request id " + std::to_string(request_id) + " was polled done by another caller");
+    }
     Request* r = it->second.get();
-    if (wait) srv->cv_done.wait(lk, [&] { return r->state == 2; });
     if (info) {
       std::memset(info, 0, sizeof(*info));
       info->state = r->state;

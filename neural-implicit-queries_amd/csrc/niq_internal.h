@@ -105,6 +105,7 @@ inline unsigned grid_for(long long work_blocks, long long persist) {
 using namespace niqi;
 
 struct niq_model {
+  uint64_t serial = 0;         // unique per niq_model_create in this process
   int n_layers = 0;
   int in_dim = 0;
   int wt = 0;                  // template width
@@ -148,6 +149,51 @@ niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, in
 // Defined in api_cp_mc.hip.
 niq_status points_dev(const niq_model* m, const float* x, const int* n_dev, int mult, long long max_n, float* out,
                       hipStream_t s);
+// The checks of niq_cast_rays (everything validated before any work is
+// enqueued; *wt, *wt16 receive the template widths), and the cast over a
+// checked request with its workspace at `base`.  Defined in api_rays.hip.
+niq_status cast_check(const niq_model* const* models, int n_models, const niq_eval_cfg* cfg, const float* roots,
+                      const float* dirs, int64_t n, const niq_cast_opts* o, const float* out_t,
+                      const int32_t* out_hit_id, const int32_t* out_count, int* wt, int* wt16,
+                      bool allow_rows = true);
+niq_status cast_run(const niq_model* const* models, int n_models, const niq_eval_cfg* cfg, const float* roots,
+                    const float* dirs, int64_t n, const niq_cast_opts* o, float* out_t, int32_t* out_hit_id,
+                    int32_t* out_count, unsigned char* base, int wt, int wt16, int64_t* n_evals, hipStream_t s);
+// One request of a cast over a table of ray groups: its functions, rays,
+// options and outputs; n_evals receives its N_evals.
+struct CastGroupReq {
+  const niq_model* const* models;
+  int n_models;
+  const float* roots;
+  const float* dirs;
+  int64_t n;
+  niq_cast_opts opts;
+  float* out_t;
+  int32_t* out_hit;
+  int32_t* out_count;
+  int64_t n_evals;
+};
+constexpr int kMaxRayGroups = 64;
+
+// Workspace layout of a group cast: per group a work counter and N_evals,
+// then the per-group histograms (everything below zero_end starts at zero),
+// the FP32 and precision-mode model tables and the group table.
+struct CastLayout {
+  size_t hist[kMaxRayGroups];
+  size_t zero_end, models, models16, groups, total;
+};
+CastLayout cast_layout(const CastGroupReq* reqs, int ng, bool with16);
+// One persistent cast over ng checked requests that share `cfg` (mode,
+// precision, variants) and the template widths wt, wt16; synchronises.
+// Defined in api_rays.hip.
+niq_status cast_run_groups(CastGroupReq* reqs, int ng, const niq_eval_cfg* cfg, unsigned char* base, int wt,
+                           int wt16, hipStream_t s);
+// cast_rays in the growing-row modes: the reference's lock-step loop over
+// the live rays with the rows classifier (stream-ordered scratch, one host
+// read per iteration).  Defined in api_rays_rows.hip.
+niq_status cast_rays_rows(const niq_model* const* models, int n_models, const niq_eval_cfg* cfg, const float* roots,
+                          const float* dirs, int64_t n, const niq_cast_opts* o, float* out_t, int32_t* out_hit_id,
+                          int32_t* out_count, int64_t* n_evals, hipStream_t s);
 // niq_tree_enqueue with the root box optionally read from the device (6
 // floats, lower then upper), so that one captured build serves any root box.
 // Defined in api_tree.hip.

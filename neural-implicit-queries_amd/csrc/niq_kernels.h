@@ -31,6 +31,20 @@ template <> struct TileCfgWide<512> { static constexpr int NB = 16, NW = 16; };
 
 constexpr int kLdsLimit = 160 * 1024;
 
+// Ray-caster tiles (eval cfg ray_variant): ray slots per workgroup (NB),
+// waves (NW) and the waves per SIMD the register budget is set for (OCC,
+// 0 = what LDS admits).  Variant 0 is TileCfg; 1 keeps its shape with a
+// 256-register budget (two waves per SIMD: no spills, fewer resident
+// workgroups); 2 takes 16-slot tiles at widths up to 128 and 32-slot tiles
+// at width 256, also with 256 registers.  Every variant sums each (ray,
+// feature) in the same order, so the results are bit-identical.
+template <int WMAX, int VAR> struct RayTile {
+  static constexpr int NB = TileCfg<WMAX>::NB, NW = TileCfg<WMAX>::NW, OCC = VAR == 0 ? 0 : 2;
+};
+template <> struct RayTile<64, 2> { static constexpr int NB = 16, NW = 4, OCC = 2; };
+template <> struct RayTile<128, 2> { static constexpr int NB = 16, NW = 4, OCC = 2; };
+template <> struct RayTile<256, 2> { static constexpr int NB = 32, NW = 8, OCC = 2; };
+
 template <int R, int NB, int WMAX>
 struct Fits { static constexpr bool value = (size_t)R * NB * WMAX * 4 <= (size_t)kLdsLimit; };
 
@@ -522,30 +536,43 @@ k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __res
 }
 
 // --------------------------------------------------------- ray casting
-struct RayArgs {
+// One group of rays of a cast: its functions (models[model0 .. model0 +
+// n_models)), options, inputs and outputs.  A cast takes a table of groups;
+// rays of different groups never share a tile, so groups may differ in
+// models and options (not in range-analysis mode or template width).
+struct RayGroup {
   const float* roots;      // (n, 3)
   const float* dirs;       // (n, 3)
   long long n;
-  int n_models;
+  int model0, n_models;
   float hit_eps, max_dist, safety, grow, shrink, init_size;
   int n_max_step, n_substeps;
   float* out_t; int* out_hit; int* out_count;
-  int* next_ray;           // work counter, zero on entry
+  unsigned long long* next_ray;  // work counter, zero on entry
   int* iter_hist;          // (n_max_step + 2,) rays by iterations lived, nullable
   float lipschitz;         // weak-SDF form (KIND 2): Lipschitz bound of f
+};
+
+struct RayArgs {
+  const RayGroup* groups;
+  int n_groups;
 };
 
 // Persistent certified-step ray caster (src/queries.py:39-175).  A workgroup
 // owns NB ray slots; every step it bounds the next segment of each live ray
 // and samples the two points of the hit test, for each function in turn.
-// Terminated slots are refilled from a global counter, so the tile stays full
-// and no host round trip happens until all rays are done.  Slot state is
-// replicated in lanes 0..NB-1 of every wave (each wave computes the same
-// update), and per-slot results are exchanged through LDS.  KIND 2 is the
-// weak-SDF form: three point planes (segment centre, start, start + hit_eps)
-// and the segment radius |half_vec| kept in LDS per slot.
-template <int KIND, int V, int NB, int NW, int WMAX>
-__global__ void __launch_bounds__(NW * 64, (LdsOcc<Planes<KIND, V, KIND == 2 ? 3 : 2>::R, NB, NW, WMAX>::value))
+// Terminated slots are refilled from the current group's counter, so the tile
+// stays full and no host round trip happens until all rays are done; once a
+// group has no rays left and the tile has drained, the workgroup moves to the
+// next group (starting from group blockIdx mod n_groups) until it has seen
+// every group run dry.  Slot state is replicated in lanes 0..NB-1 of every
+// wave (each wave computes the same update), and per-slot results are
+// exchanged through LDS.  KIND 2 is the weak-SDF form: three point planes
+// (segment centre, start, start + hit_eps) and the segment radius |half_vec|
+// kept in LDS per slot.
+template <int KIND, int V, int NB, int NW, int WMAX,
+          int OCC = LdsOcc<Planes<KIND, V, KIND == 2 ? 3 : 2>::R, NB, NW, WMAX>::value>
+__global__ void __launch_bounds__(NW * 64, OCC)
 k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
   using PL = Planes<KIND, V, KIND == 2 ? 3 : 2>;
   constexpr int R = PL::R;
@@ -553,41 +580,54 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
   constexpr int NT = NW * 64;
   __shared__ float X[R * NB * WMAX];
   __shared__ float Rd[KIND == 2 ? NB : 1];
-  int* Xi = reinterpret_cast<int*>(X);
+  // the refill's 64-bit base is passed through the first 8 bytes of the planes
+  long long* s_base = reinterpret_cast<long long*>(X);
   const int lane = threadIdx.x & 63;
   const bool slot_lane = lane < NB;
 
   int rid = -1, cnt = 0, iters = 0;
   float t = 0.f, step = 0.f;
   float ro[3] = {0.f, 0.f, 0.f}, di[3] = {0.f, 0.f, 0.f};
+  int gi = blockIdx.x % a.n_groups, seen = 0;
+  RayGroup G = a.groups[gi];
+  bool exhausted = false;   // this group's rays are all claimed
 
   while (true) {
     // ---- refill dead slots (identical in every wave)
     const unsigned long long dead = __ballot(slot_lane && rid < 0);
     const int n_dead = __popcll(dead);
     __syncthreads();
-    if (threadIdx.x == 0) Xi[0] = n_dead ? atomicAdd(a.next_ray, n_dead) : 0;
+    if (threadIdx.x == 0)
+      *s_base = (n_dead && !exhausted) ? (long long)atomicAdd(G.next_ray, (unsigned long long)n_dead) : G.n;
     __syncthreads();
-    const int base = Xi[0];
+    const long long base = *s_base;
+    if (base + n_dead >= G.n) exhausted = true;
     if (slot_lane && rid < 0) {
       const int rank = __popcll(dead & ((1ull << lane) - 1ull));
-      const long long r = (long long)base + rank;
-      if (r < a.n) {
+      const long long r = base + rank;
+      if (r >= 0 && r < G.n) {
         rid = (int)r;
-        t = 0.f; step = a.init_size; cnt = 0; iters = 0;
-        for (int k = 0; k < 3; ++k) { ro[k] = a.roots[3 * r + k]; di[k] = a.dirs[3 * r + k]; }
+        t = 0.f; step = G.init_size; cnt = 0; iters = 0;
+        for (int k = 0; k < 3; ++k) { ro[k] = G.roots[3 * r + k]; di[k] = G.dirs[3 * r + k]; }
       }
     }
     const bool any_live = __ballot(slot_lane && rid >= 0) != 0ull;
-    if (!any_live) break;   // uniform: every wave holds the same slot state
+    if (!any_live) {   // uniform: every wave holds the same slot state
+      // drained and the group has no rays left: on to the next group
+      if (++seen >= a.n_groups) break;
+      gi = gi + 1 == a.n_groups ? 0 : gi + 1;
+      G = a.groups[gi];
+      exhausted = false;
+      continue;
+    }
 
     // ---- one iteration = n_substeps substeps (src/queries.py:94-111)
     bool is_hit = false;
     int hit_id = 0, nsteps = 0;
-    for (int sub = 0; sub < a.n_substeps; ++sub) {
+    for (int sub = 0; sub < G.n_substeps; ++sub) {
       bool can_step = !is_hit;
       nsteps += is_hit ? 0 : 1;
-      for (int f = 0; f < a.n_models; ++f) {
+      for (int f = 0; f < G.n_models; ++f) {
         __syncthreads();   // previous readers of X are done
         for (int e = threadIdx.x; e < R * NB * 16; e += NT) {
           const int k = e & 15;
@@ -602,7 +642,7 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
             const float ps = ro[k] + t * di[k];
             const float hv = hs * di[k];
             const float pm = ps + hv;
-            const float pe = ro[k] + (t + a.hit_eps) * di[k];
+            const float pe = ro[k] + (t + G.hit_eps) * di[k];
             float vals[R];
             for (int p = 0; p < R; ++p) vals[p] = 0.f;
             vals[0] = pm;
@@ -619,10 +659,10 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
             Rd[b] = sqrtf(__fmul_rn(nv, nv));
           }
         }
-        mlp_forward<PL, KIND, V, NB, NW, WMAX>(models[f], X, [&](const float* v, int b) {
+        mlp_forward<PL, KIND, V, NB, NW, WMAX>(models[G.model0 + f], X, [&](const float* v, int b) {
           int code;
           if constexpr (KIND == 2) {
-            code = sdf_code(v[0], Rd[b], a.lipschitz, 0.f);
+            code = sdf_code(v[0], Rd[b], G.lipschitz, 0.f);
           } else {
             float lo, hi, scale;
             final_bounds<KIND, V>(v, lo, hi, scale);
@@ -642,23 +682,23 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
           is_hit = is_hit || this_hit;
         }
       }
-      const float this_step = can_step ? step : a.hit_eps;
-      if (!is_hit) t = t + this_step * a.safety;
-      step = can_step ? step * a.grow : step * a.shrink;
-      step = fmaxf(step, a.hit_eps);
+      const float this_step = can_step ? step : G.hit_eps;
+      if (!is_hit) t = t + this_step * G.safety;
+      step = can_step ? step * G.grow : step * G.shrink;
+      step = fmaxf(step, G.hit_eps);
     }
 
     // ---- termination (src/queries.py:113-129)
     if (slot_lane && rid >= 0) {
       cnt += nsteps;
       ++iters;
-      const bool term = is_hit || (t > a.max_dist) || (cnt >= a.n_max_step);
+      const bool term = is_hit || (t > G.max_dist) || (cnt >= G.n_max_step);
       if (term) {
         if (threadIdx.x < 64) {   // wave 0 writes
-          a.out_t[rid] = t;
-          a.out_hit[rid] = hit_id;
-          a.out_count[rid] = cnt;
-          if (a.iter_hist) atomicAdd(&a.iter_hist[min(iters, a.n_max_step + 1)], 1);
+          G.out_t[rid] = t;
+          G.out_hit[rid] = hit_id;
+          G.out_count[rid] = cnt;
+          if (G.iter_hist) atomicAdd(&G.iter_hist[min(iters, G.n_max_step + 1)], 1);
         }
         rid = -1;
       }

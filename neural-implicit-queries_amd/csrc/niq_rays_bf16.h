@@ -2,7 +2,7 @@
 // Persistent certified-step ray caster in precision mode: the step's segment
 // box is bounded by the rigorous BF16-MFMA range form (FP64 final test), and
 // the two hit-test points ride beside it as plain rows of the same split
-// products.  Slot handling, stepping and termination are those of
+// products.  Slot handling, groups, stepping and termination are those of
 // k_cast_rays (src/queries.py:39-175).
 #include "niq_bf16.h"
 #include "niq_kernels.h"
@@ -25,41 +25,54 @@ k_cast_rays_bf16(const DevModelBf16* __restrict__ models, RayArgs a) {
   constexpr int NT = NW * 64;
   __shared__ __attribute__((aligned(16))) unsigned short Xs[RP::NPT * NB * WMAX];
   float* Xf = reinterpret_cast<float*>(Xs);
-  int* Xi = reinterpret_cast<int*>(Xs);
+  // the refill's 64-bit base is passed through the first 8 bytes of the planes
+  long long* s_base = reinterpret_cast<long long*>(Xs);
   const int lane = threadIdx.x & 63;
   const bool slot_lane = lane < NB;
 
   int rid = -1, cnt = 0, iters = 0;
   float t = 0.f, step = 0.f;
   float ro[3] = {0.f, 0.f, 0.f}, di[3] = {0.f, 0.f, 0.f};
+  int gi = blockIdx.x % a.n_groups, seen = 0;
+  RayGroup G = a.groups[gi];
+  bool exhausted = false;   // this group's rays are all claimed
 
   while (true) {
     // ---- refill dead slots (identical in every wave)
     const unsigned long long dead = __ballot(slot_lane && rid < 0);
     const int n_dead = __popcll(dead);
     __syncthreads();
-    if (threadIdx.x == 0) Xi[0] = n_dead ? atomicAdd(a.next_ray, n_dead) : 0;
+    if (threadIdx.x == 0)
+      *s_base = (n_dead && !exhausted) ? (long long)atomicAdd(G.next_ray, (unsigned long long)n_dead) : G.n;
     __syncthreads();
-    const int base = Xi[0];
+    const long long base = *s_base;
+    if (base + n_dead >= G.n) exhausted = true;
     if (slot_lane && rid < 0) {
       const int rank = __popcll(dead & ((1ull << lane) - 1ull));
-      const long long r = (long long)base + rank;
-      if (r < a.n) {
+      const long long r = base + rank;
+      if (r >= 0 && r < G.n) {
         rid = (int)r;
-        t = 0.f; step = a.init_size; cnt = 0; iters = 0;
-        for (int k = 0; k < 3; ++k) { ro[k] = a.roots[3 * r + k]; di[k] = a.dirs[3 * r + k]; }
+        t = 0.f; step = G.init_size; cnt = 0; iters = 0;
+        for (int k = 0; k < 3; ++k) { ro[k] = G.roots[3 * r + k]; di[k] = G.dirs[3 * r + k]; }
       }
     }
     const bool any_live = __ballot(slot_lane && rid >= 0) != 0ull;
-    if (!any_live) break;   // uniform: every wave holds the same slot state
+    if (!any_live) {   // uniform: every wave holds the same slot state
+      // drained and the group has no rays left: on to the next group
+      if (++seen >= a.n_groups) break;
+      gi = gi + 1 == a.n_groups ? 0 : gi + 1;
+      G = a.groups[gi];
+      exhausted = false;
+      continue;
+    }
 
     bool is_hit = false;
     int hit_id = 0, nsteps = 0;
-    for (int sub = 0; sub < a.n_substeps; ++sub) {
+    for (int sub = 0; sub < G.n_substeps; ++sub) {
       bool can_step = !is_hit;
       nsteps += is_hit ? 0 : 1;
-      for (int f = 0; f < a.n_models; ++f) {
-        const DevModelBf16& M = models[f];
+      for (int f = 0; f < G.n_models; ++f) {
+        const DevModelBf16& M = models[G.model0 + f];
         __syncthreads();   // previous readers of Xs are done
         // layer-0 inputs: features 0..2 from the slot, 3..31 zero
         for (int e = threadIdx.x; e < RP::NPT * NB * 32; e += NT) {
@@ -74,7 +87,7 @@ k_cast_rays_bf16(const DevModelBf16* __restrict__ models, RayArgs a) {
             const float ps = ro[k] + t * di[k];
             const float hv = hs * di[k];
             const float pm = ps + hv;
-            const float pe = ro[k] + (t + a.hit_eps) * di[k];
+            const float pe = ro[k] + (t + G.hit_eps) * di[k];
             float s[V + 1];
             float err = 0.f;
             s[0] = live ? pm : 0.f;
@@ -110,22 +123,22 @@ k_cast_rays_bf16(const DevModelBf16* __restrict__ models, RayArgs a) {
           is_hit = is_hit || this_hit;
         }
       }
-      const float this_step = can_step ? step : a.hit_eps;
-      if (!is_hit) t = t + this_step * a.safety;
-      step = can_step ? step * a.grow : step * a.shrink;
-      step = fmaxf(step, a.hit_eps);
+      const float this_step = can_step ? step : G.hit_eps;
+      if (!is_hit) t = t + this_step * G.safety;
+      step = can_step ? step * G.grow : step * G.shrink;
+      step = fmaxf(step, G.hit_eps);
     }
 
     if (slot_lane && rid >= 0) {
       cnt += nsteps;
       ++iters;
-      const bool term = is_hit || (t > a.max_dist) || (cnt >= a.n_max_step);
+      const bool term = is_hit || (t > G.max_dist) || (cnt >= G.n_max_step);
       if (term) {
         if (threadIdx.x < 64) {
-          a.out_t[rid] = t;
-          a.out_hit[rid] = hit_id;
-          a.out_count[rid] = cnt;
-          if (a.iter_hist) atomicAdd(&a.iter_hist[min(iters, a.n_max_step + 1)], 1);
+          G.out_t[rid] = t;
+          G.out_hit[rid] = hit_id;
+          G.out_count[rid] = cnt;
+          if (G.iter_hist) atomicAdd(&G.iter_hist[min(iters, G.n_max_step + 1)], 1);
         }
         rid = -1;
       }

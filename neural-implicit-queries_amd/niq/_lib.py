@@ -49,7 +49,7 @@ class EvalCfg(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("precision", ctypes.c_int),
                 ("truncate_count", ctypes.c_int), ("n_append", ctypes.c_int),
                 ("recheck_tol", ctypes.c_float), ("variant", ctypes.c_int),
-                ("sdf_lipschitz", ctypes.c_float), ("reserved", ctypes.c_int)]
+                ("sdf_lipschitz", ctypes.c_float), ("ray_variant", ctypes.c_int)]
 
 
 class TreeParams(ctypes.Structure):

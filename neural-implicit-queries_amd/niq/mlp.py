@@ -7,6 +7,7 @@ keeps that format unchanged (load/save, spec builders) and lowers an op list
 to the dense-layer list that the gfx950 kernels run (``lower``), uploading it
 once per params dict (``device_model``).
 """
+import collections
 import ctypes
 import threading
 
@@ -232,18 +233,46 @@ def torch_init_device():
 
 
 _cache_lock = threading.Lock()
-_cache = {}
+_cache = collections.OrderedDict()   # id(params) -> (params, fingerprint, DeviceModel), LRU order
+CACHE_MAX_MODELS = 64                 # resident uploads kept for reuse
+CACHE_MAX_BYTES = 8 << 30             # ... and their device bytes
+
+
+def _fingerprint(params):
+    """What identifies the uploaded weights of a params dict: its keys and the
+    identity, shape and dtype of each array.  Replacing an entry changes it;
+    writing into an array in place does not (params are treated as immutable,
+    as JAX arrays are in the reference)."""
+    return tuple((k, id(v), getattr(v, "shape", None), str(getattr(v, "dtype", ""))) for k, v in params.items())
 
 
 def device_model(params):
-    """The uploaded model for ``params`` (cached by dict identity; params are
-    treated as immutable, as JAX arrays are in the reference)."""
+    """The uploaded model for ``params``, cached by dict identity and content
+    fingerprint.  The cache is bounded (``CACHE_MAX_MODELS`` uploads and
+    ``CACHE_MAX_BYTES`` of device memory, least recently used evicted); an
+    evicted model's device memory is freed once no caller holds it."""
     key = id(params)
+    fp = _fingerprint(params)
     with _cache_lock:
         ent = _cache.get(key)
-        if ent is not None and ent[0] is params:
-            return ent[1]
+        if ent is not None and ent[0] is params and ent[1] == fp:
+            _cache.move_to_end(key)
+            return ent[2]
     m = DeviceModel(params)
     with _cache_lock:
-        _cache[key] = (params, m)
+        _cache[key] = (params, fp, m)
+        _cache.move_to_end(key)
+        total = sum(e[2].device_bytes for e in _cache.values())
+        while len(_cache) > 1 and (len(_cache) > CACHE_MAX_MODELS or total > CACHE_MAX_BYTES):
+            _, (_, _, old) = _cache.popitem(last=False)
+            total -= old.device_bytes
     return m
+
+
+def forget_device_model(params):
+    """Drop the cached upload of ``params`` (its device memory is freed once no
+    caller holds the model)."""
+    with _cache_lock:
+        ent = _cache.get(id(params))
+        if ent is not None and ent[0] is params:
+            del _cache[id(params)]

@@ -105,3 +105,60 @@ def config_model(name):
 def flops_per_box(dims, rows):
     """Algorithmic FLOPs of one range-analysis forward: sum 2 * rows * in * out."""
     return sum(2 * rows * dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+
+
+# The serving mix: eight resident MLPs with seeds 10-17, each shape drawn
+# (default_rng(0)) from the three single-GPU shapes above.
+SERVE_SHAPES = (("c1 3->32x3->1 relu", [32] * 3, "relu", "he"),
+                ("c2 3->256x6->1 relu", [256] * 6, "relu", "he"),
+                ("c3 3->128x8->1 tanh", [128] * 8, "tanh", "xavier"))
+
+
+def serving_models(seed=0):
+    """[(shape name, params)] of the eight resident serving models, and the
+    generator the shapes were drawn from (for drawing the request trace)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(10, 18):
+        name, hidden, act, init = SERVE_SHAPES[int(rng.integers(0, 3))]
+        out.append((name, random_mlp(hidden, activation=act, init=init, seed=s)))
+    return out, rng
+
+
+def concurrent_tree_models(n_models=16, seed=1):
+    """The 16 TanH MLPs of the concurrent-tree workload (Xavier init, seeds
+    20-35): widths from {32, 64, 128, 256} and 3-8 hidden layers drawn from
+    default_rng(seed).  Returns [(width, hidden layers, params)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_models):
+        w = int(rng.choice([32, 64, 128, 256]))
+        nl = int(rng.integers(3, 9))
+        out.append((w, nl, random_mlp([w] * nl, activation="tanh", init="xavier", seed=20 + i)))
+    return out
+
+
+def flops_per_box_affine_all(dims):
+    """Algorithmic FLOPs of one affine_all forward: layer l contracts its
+    1 + v_l rows (base and the error terms), with v_0 the input dimension and
+    v growing by the layer's width at every hidden activation."""
+    v = dims[0]
+    total = 0
+    for i in range(len(dims) - 1):
+        total += 2 * (1 + v) * dims[i] * dims[i + 1]
+        if i + 2 < len(dims):
+            v += dims[i + 1]
+    return total
+
+
+def error_bytes_per_box_affine_all(dims):
+    """HBM bytes of one affine_all forward whose error-term matrix lives in
+    memory: each hidden layer reads its input rows and writes its output rows
+    (v_l x width floats each)."""
+    v = dims[0]
+    total = 0
+    for i in range(len(dims) - 2):
+        v_out = v + dims[i + 1]
+        total += 4 * (v * dims[i] + v_out * dims[i + 1])
+        v = v_out
+    return total

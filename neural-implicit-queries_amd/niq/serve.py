@@ -57,7 +57,8 @@ class Request:
             return False
         self.info = {"status": info.status, "queue_ms": info.queue_ms, "latency_ms": info.latency_ms,
                      "n_evals": int(info.n_evals), "n_batched": int(info.n_batched),
-                     "graph_replay": bool(info.graph_replay), "n_processed": int(info.n_processed)}
+                     "graph_replay": bool(info.graph_replay), "graph_reused": info.graph_replay == 2,
+                     "n_processed": int(info.n_processed)}
         try:
             _lib.check(info.status)
             self._result = self._finish(self)

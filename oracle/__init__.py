@@ -20,7 +20,13 @@ How each module is pinned (no reference golden vectors exist for this path):
   * ``mlp``      - key grammar checked against the reference's sample ``.npz``
                    files; point evaluation checked against closed forms.
   * ``affine``   - exactness on activation-free MLPs (closed form), Chebyshev
-                   ReLU special cases, sampled soundness, FP32 vs FP64.
+                   ReLU special cases, sampled soundness, FP32 vs FP64; the
+                   sin slope against its definition (midrange of cos over the
+                   interval, dense sampling) and closed forms on [0, pi/2],
+                   [-pi/2, pi/2], [0, pi]; truncate_affine's kept rows and
+                   affine_append's kept deltas against exhaustive subset
+                   search, and both against affine_all when nothing is
+                   dropped.
   * ``slope``    - exactness of the slope bound on activation-free MLPs,
                    sampled soundness.
   * ``tree``     - brute-force ancestor walk on the finest grid.
@@ -38,11 +44,18 @@ How each module is pinned (no reference golden vectors exist for this path):
                    unit-normal plane; sign-constancy of determinate boxes.
   * ``render``   - plane normals exactly; float64 gradients of a smooth net;
                    the matcap lookup against scipy.ndimage.map_coordinates.
+  * ``margins``  - not a restatement: it evaluates each decision of the
+                   queries in FP32 and FP64 and marks the ones within the
+                   FP32 rounding band of their threshold (1e-6 relative plus
+                   4x the FP32-FP64 difference), so tests can require every
+                   GPU result that differs from the oracle to carry a mark.
 The golden fixtures under tests/golden/ (written by make_golden.py) freeze the
 outputs of these modules for one seeded model, so later changes are caught.
 The TanH rule has no reference counterpart (the reference implements relu,
-elu and sin only); it is pinned by sampled soundness and FP64 invariants only.
+elu and sin only); it is pinned by sampled soundness, FP64 invariants and its
+tangent limit only.
 Parity unpinned (no available check fixes them): the FP32 bits of XLA's dot
-and reduction order, XLA's transcendental ulps, and the tie order of argsort /
-top_k in affine_truncate / affine_append (ties are broken lowest index first).
+and reduction order, XLA's transcendental ulps, and the order argsort / top_k
+give exact ties in affine_truncate / affine_append (broken lowest index first
+here; the kept set itself is pinned).
 """

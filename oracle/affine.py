@@ -73,6 +73,33 @@ def coordinates_in_general_box(ctx, center, vecs):
     return base, aff, err
 
 
+# Selection-gap log: when set to a list, truncate_affine and affine_append
+# append, per call, each box's relative gap between the last row kept and the
+# first row dropped (the margin of that selection; inf when nothing is
+# dropped), so a test can tell cells whose row choice is a near tie.
+_gap_log = None
+
+
+def log_selection_gaps(log):
+    """Start (a list) or stop (None) recording selection gaps."""
+    global _gap_log
+    _gap_log = log
+
+
+def _record_gap(vals_desc, k):
+    """vals_desc (N, m) sorted decreasing; gap between entries k-1 and k."""
+    if _gap_log is None:
+        return
+    if k <= 0 or k >= vals_desc.shape[1]:
+        _gap_log.append(np.full(vals_desc.shape[0], np.inf))
+        return
+    a = vals_desc[:, k - 1].astype(np.float64)
+    b = vals_desc[:, k].astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = np.where(a > 0, (a - b) / a, np.inf)
+    _gap_log.append(g)
+
+
 def truncate_affine(ctx, base, aff, err):
     """src/affine.py:127-162 ('absolute' policy).
 
@@ -88,6 +115,7 @@ def truncate_affine(ctx, base, aff, err):
         raise ValueError("only the 'absolute' truncate policy is supported")
     mags = np.sum(np.abs(aff), axis=-1)                      # (N, v)
     order = np.argsort(-mags, axis=-1, kind="stable")        # decreasing
+    _record_gap(np.take_along_axis(mags, order, axis=1), n_keep)
     aff = np.take_along_axis(aff, order[:, :, None], axis=1)
     aff_keep = aff[:, :n_keep, :]
     aff_drop = aff[:, n_keep:, :]
@@ -116,7 +144,9 @@ def apply_linear_approx(ctx, base, aff, err, alpha, beta, delta):
         err = alpha * err
         n = ctx.n_append
         # lax.top_k: largest first, lower index first on ties
-        order = np.argsort(-delta, axis=-1, kind="stable")[:, :n]
+        full = np.argsort(-delta, axis=-1, kind="stable")
+        _record_gap(np.take_along_axis(delta, full, axis=1), n)
+        order = full[:, :n]
         keep_vals = np.take_along_axis(delta, order, axis=-1)
         N, w = delta.shape
         new_aff = np.zeros((N, n, w), dtype=delta.dtype)

@@ -30,7 +30,18 @@ def all_same_sign(vals):
 
 
 def closest_point(func, lower, upper, query_points, eps=0.001, batch_process_size=2048,
-                  max_rounds=None):
+                  max_rounds=None, track_near=False):
+    """Returns (dist (Q,), loc (Q, 3), info).  With ``track_near`` (func is a
+    ``margins.TwinFunc``) info["near"] marks the queries one of whose node
+    decisions (classification, sample signs) lay near its threshold,
+    info["near_round"] is the first round in which any query's did, and
+    info["last_round"] the last round that processed a node of each query.
+    Queries share the stack: a flipped decision changes how many nodes are
+    pushed, hence which nodes later rounds pop together and the minimum each
+    pruning test sees, so a query processed after ``near_round`` may change
+    without a near decision of its own.  With batch_process_size above the
+    largest stack every round pops the whole stack and no query depends on
+    another's decisions."""
     B = batch_process_size
     q = np.asarray(query_points, f32)
     Q = q.shape[0]
@@ -45,6 +56,9 @@ def closest_point(func, lower, upper, query_points, eps=0.001, batch_process_siz
     offsets = np.concatenate((np.zeros((1, d)), np.eye(d), -np.eye(d)), 0).astype(f32)
     rounds = 0
     n_processed = 0
+    near = np.zeros(Q, bool)
+    near_round = None
+    last_round = np.full(Q, -1, np.int64)
     while top > 0:
         pop = max(top - B, 0)
         n = top - pop
@@ -64,8 +78,16 @@ def closest_point(func, lower, upper, query_points, eps=0.001, batch_process_siz
         is_small = width < eps_cube_width
         samples = center[:, None, :] + ext[:, None, :] * offsets[None, :, :]
         code = func.classify_box(blo, bup)
+        if track_near:
+            nb = func.last_near.copy()
         outside = (code == SIGN_NEGATIVE) | (code == SIGN_POSITIVE)
         vals = func(samples.reshape(-1, d)).reshape(n, 7)
+        if track_near:
+            nb |= func.last_near.reshape(n, 7).any(axis=1)
+            near[bq[nb]] = True
+            if near_round is None and nb.any():
+                near_round = rounds
+            last_round[bq] = rounds
         spans = ~all_same_sign(vals)
         this_dist = np.where(spans, max_dist, f32(np.inf)).astype(f32)
         needs = (~outside) & (~is_small) & (dc < qmin)
@@ -103,4 +125,9 @@ def closest_point(func, lower, upper, query_points, eps=0.001, batch_process_siz
         rounds += 1
         if max_rounds is not None and rounds >= max_rounds:
             break
-    return min_dist, min_loc, {"rounds": rounds, "n_processed": n_processed}
+    info = {"rounds": rounds, "n_processed": n_processed}
+    if track_near:
+        info["near"] = near
+        info["near_round"] = near_round
+        info["last_round"] = last_round
+    return min_dist, min_loc, info

@@ -69,9 +69,12 @@ def subdivide(rng):
     return A, B
 
 
-def cast_rays_frustum(funcs, root, look, up, left, fov_x, fov_y, res_x, res_y, opts):
+def cast_rays_frustum(funcs, root, look, up, left, fov_x, fov_y, res_x, res_y, opts, track_near=False):
     """src/queries.py:454-587.  Returns (t (res_x, res_y), hit_id, count, N_evals)
-    indexed [x, y] like the reference's output arrays."""
+    indexed [x, y] like the reference's output arrays.  With ``track_near``
+    (the funcs are ``margins.TwinFunc``) a fifth (res_x, res_y) result marks
+    the pixels whose frustum, or one of its ancestors, took a decision near
+    its threshold."""
     if not isinstance(funcs, (tuple, list)):
         funcs = (funcs,)
     root = np.asarray(root, f32)
@@ -100,7 +103,8 @@ def cast_rays_frustum(funcs, root, look, up, left, fov_x, fov_y, res_x, res_y, o
     t = np.zeros(n0, f32)
     step = np.full(n0, f32(opts["interval_init_size"] * opts["max_dist"]), f32)
     cnt = np.zeros(n0, f32)
-    fin = []    # (range, t, hit_id, count) of terminated frusta
+    near = np.zeros(n0, bool)
+    fin = []    # (range, t, hit_id, count, near) of terminated frusta
     bucket, empty = n0, n0
     N_evals = 0
     it = 0
@@ -123,10 +127,18 @@ def cast_rays_frustum(funcs, root, look, up, left, fov_x, fov_y, res_x, res_y, o
             vecs = np.stack((cvec, right_front, up_front), axis=1)
             for fid, func in enumerate(funcs, start=1):
                 code = func.classify_general_box(center, vecs)
+                if track_near:
+                    near |= func.last_near
                 can_step &= (code == SIGN_POSITIVE) | (code == SIGN_NEGATIVE)
                 p0 = (root[None, :] + t[:, None] * mid).astype(f32)
                 p1 = (root[None, :] + (t + hit_eps)[:, None] * mid).astype(f32)
-                this_hit = np.sign(func(p0)) != np.sign(func(p1))
+                v0 = func(p0)
+                if track_near:
+                    near |= func.last_near
+                v1 = func(p1)
+                if track_near:
+                    near |= func.last_near
+                this_hit = np.sign(v0) != np.sign(v1)
                 hit_id = np.where(this_hit, fid, hit_id).astype(np.int32)
                 is_hit |= this_hit
             this_step = np.where(can_step, step, hit_eps * single.astype(f32)).astype(f32)
@@ -138,7 +150,7 @@ def cast_rays_frustum(funcs, root, look, up, left, fov_x, fov_y, res_x, res_y, o
         cnt = (cnt + nsteps.astype(f32) * (f32(1.0) / area)).astype(f32)
         is_hit = is_hit & (area == 1)
         term = is_hit | (t > max_dist) | (it >= n_max)             # src/queries.py:333-336
-        for arr in zip(rng[term], t[term], hit_id[term], cnt[term]):
+        for arr in zip(rng[term], t[term], hit_id[term], cnt[term], near[term]):
             fin.append(arr)
         keep = ~term
         # refinement test on the updated t and step (src/queries.py:348-362)
@@ -147,7 +159,7 @@ def cast_rays_frustum(funcs, root, look, up, left, fov_x, fov_y, res_x, res_y, o
         can_subd = (rng[:, 2] > rng[:, 0] + 1) | (rng[:, 3] > rng[:, 1] + 1)
         needs = ((wx > refine * step) | (wy > refine * step) | demands) & can_subd & keep
         it += n_sub
-        rng, t, step, cnt, needs = rng[keep], t[keep], step[keep], cnt[keep], needs[keep]
+        rng, t, step, cnt, needs, near = rng[keep], t[keep], step[keep], cnt[keep], needs[keep], near[keep]
         n_valid, n_refine = rng.shape[0], int(needs.sum())
         if n_valid == 0:
             break
@@ -163,13 +175,18 @@ def cast_rays_frustum(funcs, root, look, up, left, fov_x, fov_y, res_x, res_y, o
         t = np.concatenate((t, t[needs]))
         step = np.concatenate((step, step[needs]))
         cnt = np.concatenate((cnt, cnt[needs]))
+        near = np.concatenate((near, near[needs]))
 
     # each finished frustum covers its pixels (src/queries.py:571-585)
     out_t = np.zeros((res_x, res_y), f32)
     out_hit = np.zeros((res_x, res_y), np.int32)
     out_count = np.zeros((res_x, res_y), np.int32)
-    for r, tv, hv, cv in fin:
+    out_near = np.zeros((res_x, res_y), bool)
+    for r, tv, hv, cv, nv in fin:
         out_t[r[0]:r[2], r[1]:r[3]] = tv
         out_hit[r[0]:r[2], r[1]:r[3]] = hv
         out_count[r[0]:r[2], r[1]:r[3]] = np.int32(cv)
+        out_near[r[0]:r[2], r[1]:r[3]] = nv
+    if track_near:
+        return out_t, out_hit, out_count, N_evals, out_near
     return out_t, out_hit, out_count, N_evals

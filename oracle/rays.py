@@ -68,12 +68,14 @@ def generate_camera_rays(eye_pos, look_dir, up_dir, res=1024, fov_deg=30.0):
     return roots, dirs
 
 
-def cast_rays(funcs, roots, dirs, opts):
+def cast_rays(funcs, roots, dirs, opts, track_near=False):
     """src/queries.py:134-175 for a tuple of oracle functions.
 
     Returns (t (N,), hit_id (N,) int32, count (N,) int32, N_evals).  The
     iteration is lock-step over all live rays exactly as the reference; the
-    bucketed array size is tracked only to reproduce its ``N_evals``."""
+    bucketed array size is tracked only to reproduce its ``N_evals``.  With
+    ``track_near`` (the funcs are ``margins.TwinFunc``) a fifth result marks
+    the rays one of whose decisions lay near its threshold."""
     if not isinstance(funcs, (tuple, list)):
         funcs = (funcs,)
     roots = np.asarray(roots, f32)
@@ -96,6 +98,7 @@ def cast_rays(funcs, roots, dirs, opts):
     count = np.zeros(N, np.int32)
     bucket = N
     N_evals = 0
+    near = np.zeros(N, bool)
     while idx.shape[0] > 0:
         r, d = roots[idx], dirs[idx]
         is_hit = np.zeros(idx.shape[0], bool)
@@ -109,10 +112,16 @@ def cast_rays(funcs, roots, dirs, opts):
                 half_vec = (f32(0.5) * step)[:, None] * d
                 pos_mid = pos_start + half_vec
                 code = func.classify_general_box(pos_mid, half_vec[:, None, :])
+                if track_near:
+                    near[idx] |= func.last_near
                 can_step &= (code == SIGN_POSITIVE) | (code == SIGN_NEGATIVE)
                 pos_eps = r + (t + hit_eps)[:, None] * d
                 v0 = func(pos_start)
+                if track_near:
+                    near[idx] |= func.last_near
                 v1 = func(pos_eps)
+                if track_near:
+                    near[idx] |= func.last_near
                 this_hit = np.sign(v0) != np.sign(v1)
                 hit_id = np.where(this_hit, fid, hit_id).astype(np.int32)
                 is_hit |= this_hit
@@ -131,4 +140,6 @@ def cast_rays(funcs, roots, dirs, opts):
         n_valid = idx.shape[0]
         if n_valid > 0 and get_next_bucket_size(n_valid) < bucket:   # src/queries.py:170-173
             bucket = get_next_bucket_size(n_valid)
+    if track_near:
+        return out_t, out_hit, out_count, N_evals, near
     return out_t, out_hit, out_count, N_evals

@@ -11,8 +11,13 @@
       with interior and exterior lists (the reference's bucketed arrays)
     ray_*: a 64x64 camera at (0, 0, 3) looking at the origin, 60 degree field
       of view, max_dist 6, other options at their defaults: roots, dirs,
-      t, hit_id, count, n_evals
-    cp_*: closest_point of 64 seeded queries, eps 0.01
+      t, hit_id, count, n_evals, and near (the rays one of whose step or hit
+      decisions lay within the FP32 rounding band of its threshold,
+      oracle/margins.py)
+    cp_*: closest_point of 64 seeded queries, eps 0.01, batches of 256 (dist,
+      loc, near per query, and the round each query was last processed in
+      and the first round with a near decision: queries share the stack);
+      cpw_*: the same with one batch for the whole stack (no coupling)
 
 The oracle follows the reference's algorithm (file:line in each oracle
 module); these fixtures freeze its outputs so that later changes to the
@@ -32,12 +37,14 @@ for p in (os.path.join(ROOT, "neural-implicit-queries_amd"), ROOT):
         sys.path.insert(0, p)
 
 from oracle import funcs as ofuncs, tree as otree, rays as orays, closest as ocp, mlp as omlp  # noqa: E402
+from oracle import margins  # noqa: E402
 from niq import models, mlp as nmlp  # noqa: E402
 
 MODEL = "c1_relu_3x32"
 MODES = ("interval", "affine_fixed", "slope_interval", "affine_all")
 LO = np.array([-1., -1., -1.], np.float32)
 UP = np.array([1., 1., 1.], np.float32)
+CP_WIDE_BATCH = 1 << 20   # above the largest closest-point stack of these queries
 
 
 def golden_inputs():
@@ -81,12 +88,18 @@ def main():
         for part in ("valid", "lower", "upper"):
             out[f"tree_{key}_{part}"] = np.asarray(tree[f"{key}_node_{part}"])
     out["tree_n_classified"] = np.int64(tree["stats"]["n_classified"])
-    t, h, c, ne = orays.cast_rays((ofuncs.OracleFunc(p, "affine_fixed"),), roots, dirs, ray_opts())
+    t, h, c, ne, near = orays.cast_rays((margins.TwinFunc(p, "affine_fixed"),), roots, dirs, ray_opts(),
+                                        track_near=True)
     out.update(ray_roots=roots, ray_dirs=dirs, ray_t=np.asarray(t, np.float32), ray_hit_id=np.asarray(h, np.int32),
-               ray_count=np.asarray(c, np.int32), ray_n_evals=np.int64(ne))
-    d, loc, _ = ocp.closest_point(ofuncs.OracleFunc(p, "affine_fixed"), LO, UP, q, eps=0.01,
-                                  batch_process_size=256)
-    out.update(cp_query=q, cp_dist=np.asarray(d, np.float32), cp_loc=np.asarray(loc, np.float32))
+               ray_count=np.asarray(c, np.int32), ray_n_evals=np.int64(ne), ray_near=near)
+    d, loc, info = ocp.closest_point(margins.TwinFunc(p, "affine_fixed"), LO, UP, q, eps=0.01,
+                                     batch_process_size=256, track_near=True)
+    out.update(cp_query=q, cp_dist=np.asarray(d, np.float32), cp_loc=np.asarray(loc, np.float32),
+               cp_near=info["near"], cp_last_round=info["last_round"],
+               cp_near_round=np.int64(-1 if info["near_round"] is None else info["near_round"]))
+    d, loc, info = ocp.closest_point(margins.TwinFunc(p, "affine_fixed"), LO, UP, q, eps=0.01,
+                                     batch_process_size=CP_WIDE_BATCH, track_near=True)
+    out.update(cpw_dist=np.asarray(d, np.float32), cpw_loc=np.asarray(loc, np.float32), cpw_near=info["near"])
     np.savez_compressed(os.path.join(HERE, "c1_golden.npz"), **out)
     print("wrote", os.path.join(HERE, "c1_golden.npz"), {k: v.shape for k, v in out.items()})
 

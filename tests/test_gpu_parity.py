@@ -16,7 +16,16 @@ Bars (written here, checked per case):
     POSITIVE/NEGATIVE unless the FP64 oracle reports the same sign;
   * tree lists: identical arrays (order included) when no near-threshold
     decision differs;
-  * rays: identical (t, hit_id, count) for all but near-threshold rays.
+  * rays, frustum pixels, closest-point distances: identical, except results
+    that the oracle marks near-threshold (oracle/margins.py: one of the ray's,
+    frustum's or query's decisions had its FP64 value within
+    1e-6 * scale + 4 |oracle32 - oracle64| of the threshold, scale being
+    |lower| + |upper| for a bound and the final layer's sum of |terms| for a
+    point value); every differing result must carry that mark, and the counts
+    of both are printed;
+  * affine_truncate / affine_append: bounds and codes as above, except cells
+    whose row selection is a near tie (the last kept and first dropped
+    magnitudes within TIE_REL, relative), counted.
 """
 import sys
 
@@ -25,7 +34,7 @@ import pytest
 import torch
 
 from oracle import funcs as ofuncs, tree as otree, rays as orays, mlp as omlp, affine as oaff
-from oracle import frustum as ofr, intersect as oisect, sampling as osamp, render as orender
+from oracle import frustum as ofr, intersect as oisect, sampling as osamp, render as orender, margins
 from niq import models, mlp as nmlp, implicit_mlp_utils, kd_tree, queries, render
 from niq import _lib as _lib_mod
 
@@ -34,11 +43,21 @@ pytestmark = pytest.mark.gpu
 LO = np.array([-1., -1., -1.], np.float32)
 UP = np.array([1., 1., 1.], np.float32)
 NEAR_REL = 1e-6
+TIE_REL = 1e-5
 
 
 def note(msg):
     print(msg, flush=True)
     sys.stdout.flush()
+
+
+def only_near(same, near, what, extra=""):
+    """Every result that differs from the oracle's must be near-threshold."""
+    same, near = np.asarray(same).reshape(-1), np.asarray(near).reshape(-1)
+    differ = ~same
+    note(f"  {what}: {int(differ.sum())} of {same.size} differ, {int(near.sum())} near-threshold, "
+         f"{int((differ & ~near).sum())} differ without the mark" + (f"; {extra}" if extra else ""))
+    assert not np.any(differ & ~near), np.nonzero(differ & ~near)[0][:10]
 
 
 def model(name):
@@ -241,13 +260,13 @@ def test_rays_match_oracle_c1():
     opts = queries.get_default_cast_opts()
     opts["max_dist"] = 6.0
     t, hid, cnt, ne = queries.cast_rays((func,), (p,), roots, dirs, opts)
-    of = ofuncs.OracleFunc(p, "affine_fixed")
-    rt, rh, rc, rne = orays.cast_rays((of,), roots, dirs, opts)
+    of = margins.TwinFunc(p, "affine_fixed")
+    rt, rh, rc, rne, near = orays.cast_rays((of,), roots, dirs, opts, track_near=True)
     t, hid, cnt = t.cpu().numpy(), hid.cpu().numpy(), cnt.cpu().numpy()
     same = (t == rt) & (hid == rh) & (cnt == rc)
-    note(f"  {same.mean() * 100:.3f}% rays identical, n_evals gpu {ne} oracle {rne}")
-    assert same.mean() >= 0.995
-    assert (hid == rh).mean() >= 0.998
+    only_near(same, near, "rays", f"n_evals gpu {ne} oracle {rne}")
+    if same.all():
+        assert ne == rne
 
 
 def test_rays_plane_closed_form():
@@ -278,9 +297,11 @@ def test_rays_two_functions():
     opts = queries.get_default_cast_opts()
     opts["max_dist"] = 6.0
     t, hid, cnt, _ = queries.cast_rays((f1, f2), (p1, p2), roots, dirs, opts)
-    rt, rh, rc, _ = orays.cast_rays((ofuncs.OracleFunc(p1), ofuncs.OracleFunc(p2)), roots, dirs, opts)
-    same = (t.cpu().numpy() == rt) & (hid.cpu().numpy() == rh)
-    assert same.mean() >= 0.99
+    rt, rh, rc, _, near = orays.cast_rays((margins.TwinFunc(p1), margins.TwinFunc(p2)), roots, dirs, opts,
+                                          track_near=True)
+    same = (t.cpu().numpy() == rt) & (hid.cpu().numpy() == rh) & (cnt.cpu().numpy() == rc)
+    only_near(same, near, "rays (two functions)")
+    assert (rh == 2).sum() > 0 and (rh == 1).sum() > 0
 
 
 # ------------------------------------------------------------ frustum rays
@@ -293,8 +314,9 @@ def frust_cam(res, eye=(0., 0., 3.)):
 @pytest.mark.parametrize("name", ["c1_relu_3x32", "small_tanh", "fox"])
 def test_frustum_matches_oracle(name):
     """Frustum casting (src/queries.py:454-587): per pixel (t, hit_id, count)
-    identical to the oracle's for >= 99 % of pixels (near-threshold FP32
-    decisions aside), and N_evals identical when every pixel is."""
+    identical to the oracle's except at pixels whose frustum took a
+    near-threshold decision (counted), and N_evals identical when every pixel
+    is."""
     note(f"[frustum] {name}")
     p = model(name)
     f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
@@ -303,12 +325,10 @@ def test_frustum_matches_oracle(name):
     opts["n_side_init"] = 8
     cam = frust_cam(64)
     t, h, c, ne = queries.cast_rays_frustum((f,), (p,), cam, opts)
-    rt, rh, rc, rne = ofr.cast_rays_frustum((ofuncs.OracleFunc(p),), *cam, opts)
+    rt, rh, rc, rne, near = ofr.cast_rays_frustum((margins.TwinFunc(p),), *cam, opts, track_near=True)
     t, h, c = t.cpu().numpy(), h.cpu().numpy(), c.cpu().numpy()
     same = (t == rt) & (h == rh) & (c == rc)
-    note(f"  {same.mean() * 100:.2f}% pixels identical; hits {int((h > 0).sum())} / {int((rh > 0).sum())}; "
-         f"N_evals {ne} / {rne}")
-    assert same.mean() >= 0.99
+    only_near(same, near, "pixels", f"hits {int((h > 0).sum())} / {int((rh > 0).sum())}; N_evals {ne} / {rne}")
     if same.all():
         assert ne == rne
 
@@ -433,7 +453,8 @@ def test_sdf_classify_matches_oracle(name):
 
 def test_sdf_rays_match_oracle():
     """Ray casting with the weak-SDF step test: the same (t, hit_id, count) as
-    the oracle's for >= 99.5 % of rays."""
+    the oracle's except for rays with a near-threshold decision (counted), per
+    ray and per frustum pixel."""
     note("[sdf rays fox 64x64]")
     p = model("fox")
     lip = 1.0
@@ -442,19 +463,17 @@ def test_sdf_rays_match_oracle():
     opts = queries.get_default_cast_opts()
     opts["max_dist"] = 6.0
     t, hid, cnt, ne = queries.cast_rays((func,), (p,), roots, dirs, opts)
-    rt, rh, rc, rne = orays.cast_rays((ofuncs.OracleFunc(p, "sdf", sdf_lipschitz=lip),), roots, dirs, opts)
+    rt, rh, rc, rne, near = orays.cast_rays((margins.TwinFunc(p, "sdf", sdf_lipschitz=lip),), roots, dirs, opts,
+                                            track_near=True)
     t, hid, cnt = t.cpu().numpy(), hid.cpu().numpy(), cnt.cpu().numpy()
     same = (t == rt) & (hid == rh) & (cnt == rc)
-    note(f"  {same.mean() * 100:.3f}% rays identical; hits {int((hid > 0).sum())} / {int((rh > 0).sum())}; "
-         f"n_evals gpu {ne} oracle {rne}")
-    assert same.mean() >= 0.995
+    only_near(same, near, "rays", f"hits {int((hid > 0).sum())} / {int((rh > 0).sum())}; n_evals {ne} / {rne}")
     f2, _ = implicit_mlp_utils.implicit_from_params(p, "sdf", sdf_lipschitz=lip)
     frt, frh, frc, fne = queries.cast_rays_frustum((f2,), (p,), frust_cam(64), dict(opts, n_side_init=8))
-    ort, orh, orc, one = ofr.cast_rays_frustum((ofuncs.OracleFunc(p, "sdf", sdf_lipschitz=lip),), *frust_cam(64),
-                                               dict(opts, n_side_init=8))
+    ort, orh, orc, one, fnear = ofr.cast_rays_frustum((margins.TwinFunc(p, "sdf", sdf_lipschitz=lip),),
+                                                      *frust_cam(64), dict(opts, n_side_init=8), track_near=True)
     fsame = (frt.cpu().numpy() == ort) & (frh.cpu().numpy() == orh) & (frc.cpu().numpy() == orc)
-    note(f"  frustum: {fsame.mean() * 100:.2f}% pixels identical; N_evals {fne} / {one}")
-    assert fsame.mean() >= 0.99
+    only_near(fsame, fnear, "frustum pixels", f"N_evals {fne} / {one}")
 
 
 # ------------------------------------------------------------ normals and shading
@@ -549,8 +568,9 @@ def test_shading_matches_oracle():
 @pytest.mark.parametrize("frustum", [False, True])
 def test_render_image_matches_oracle(frustum):
     """render_image (src/render.py:94-150) of the fox with matcap shading: the
-    same hit ids and depths as the oracle's pipeline for >= 99 % of pixels,
-    colours within 2e-3 where both agree on the hit."""
+    same hit ids and depths as the oracle's pipeline except at pixels with a
+    near-threshold decision (counted), colours within 2e-3 where both agree
+    on the hit."""
     note(f"[render_image fox frustum={frustum}]")
     p = model("fox")
     func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
@@ -567,10 +587,11 @@ def test_render_image_matches_oracle(frustum):
     roots, dirs = render.generate_camera_rays(eye, look, up, res=res, fov_deg=45.0, device=None)
     of = ofuncs.OracleFunc(p)
     if frustum:
-        rt, rh, _, _ = ofr.cast_rays_frustum((of,), eye, look, up, left, 45.0, 45.0, res, res, opts)
-        rt, rh = rt.T.reshape(-1), rh.T.reshape(-1)
+        rt, rh, _, _, near = ofr.cast_rays_frustum((margins.TwinFunc(p),), eye, look, up, left, 45.0, 45.0, res, res,
+                                                   opts, track_near=True)
+        rt, rh, near = rt.T.reshape(-1), rh.T.reshape(-1), near.T.reshape(-1)
     else:
-        rt, rh, _, _ = orays.cast_rays((of,), roots, dirs, opts)
+        rt, rh, _, _, near = orays.cast_rays((margins.TwinFunc(p),), roots, dirs, opts, track_near=True)
     pos = (roots + rt[:, None] * dirs).astype(np.float32)
     nrm = orender.outward_normals((of,), pos, rh, opts["hit_eps"])
     col = orender.shade_image("matcap_color", dirs, pos, nrm, rh, up, mats, ((0.157, 0.613, 1.0),))
@@ -580,9 +601,9 @@ def test_render_image_matches_oracle(frustum):
     # negative mixes (c_k < 0) tonemap to NaN in both pipelines, as in the reference
     ok = np.isclose(img.reshape(-1, 3), want.reshape(-1, 3), rtol=0, atol=2e-3, equal_nan=True).all(axis=1)
     err = np.nan_to_num(np.abs(img.reshape(-1, 3) - want.reshape(-1, 3))).max(axis=1)
-    note(f"  {same.mean() * 100:.2f}% pixels with the same hit and depth; {int((hid > 0).sum())} hits; "
-         f"colour error there max {err[same].max():.2e}")
-    assert same.mean() >= 0.99 and (hid > 0).sum() > 50
+    only_near(same, near, "pixels (hit and depth)",
+              f"{int((hid > 0).sum())} hits; colour error where equal max {err[same].max():.2e}")
+    assert (hid > 0).sum() > 50
     assert np.all(ok[same])
 
 
@@ -795,7 +816,7 @@ def test_bf16_mode_is_sound_and_close(name, mode):
     # tiny boxes (half-widths down to 2e-4) are included: the widening is
     # relative to |X| |W|, not to the radius, so it shows most where the true
     # radius is smallest (measured medians 1.02-1.51, the bunny's the widest);
-    # at tree-cell sizes the finest-cell test below holds it to 5 %
+    # the finest-cell test below reports it at depth-21 tree cells
     assert np.median(infl) < 1.6
 
 
@@ -836,43 +857,71 @@ def test_bf16_layerwise_tree_equals_fused():
     assert all(torch.equal(a[k], b[k]) for k in a)
 
 
-def test_bf16_tree_matches_fp32_up_to_inflation():
-    p = model("c4_relu_512x8")
+@pytest.mark.parametrize("name,depth", [("c4_relu_512x8", 12), ("fox", 18), ("bunny", 18)])
+def test_bf16_tree_matches_fp32_up_to_inflation(name, depth):
+    """The precision-mode tree against the FP32 tree and the FP64 oracle: every
+    determinate BF16 leaf (interior and exterior, sampled) is determinate with
+    the same sign under FP64, so no cell that may contain the surface is
+    dropped; the BF16 tree keeps at least the FP32 tree's unknown cells.  The
+    He-init 3->512x8->1 net has no determinate cell at this depth under either
+    precision (a full tree), so its leaves are checked on the trained nets."""
+    p = model(name)
     f32, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
     f16, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision="bf16")
-    a, sa = kd_tree.build_tree(f32, p, LO, UP, split_depth=12, with_interior_nodes=True)
-    b, sb = kd_tree.build_tree(f16, p, LO, UP, split_depth=12, with_interior_nodes=True)
-    note(f"[bf16 tree c4 depth 12] boxes fp32 {sa['n_classified']} bf16 {sb['n_classified']}, "
-         f"unknown {sa['n_unknown']} vs {sb['n_unknown']}")
-    # every BF16 interior leaf is FP64-negative (sampled)
-    m = int(b["interior_node_valid"].sum())
-    if m:
-        idx = np.random.default_rng(0).choice(m, size=min(m, 300), replace=False)
-        l = b["interior_node_lower"][:m].cpu().numpy()[idx]
-        u = b["interior_node_upper"][:m].cpu().numpy()[idx]
-        assert np.all(ofuncs.OracleFunc(p, "affine_fixed", np.float64).classify_box(l, u) == 2)
+    a, sa = kd_tree.build_tree(f32, p, LO, UP, split_depth=depth, with_interior_nodes=True, with_exterior_nodes=True)
+    b, sb = kd_tree.build_tree(f16, p, LO, UP, split_depth=depth, with_interior_nodes=True, with_exterior_nodes=True)
+    note(f"[bf16 tree {name} depth {depth}] boxes fp32 {sa['n_classified']} bf16 {sb['n_classified']}, "
+         f"unknown {sa['n_unknown']} vs {sb['n_unknown']}, interior {sb['n_interior']}, exterior {sb['n_exterior']}")
     assert sb["n_unknown"] >= sa["n_unknown"] * 0.999
+    o64 = ofuncs.OracleFunc(p, "affine_fixed", np.float64)
+    checked = 0
+    for kind, code in (("interior", 2), ("exterior", 1)):
+        m = int(b[f"{kind}_node_valid"].sum())
+        if name == "c4_relu_512x8":
+            assert m == 0 and sa[f"n_{kind}"] == 0
+            continue
+        if m == 0:
+            continue
+        idx = np.random.default_rng(0).choice(m, size=min(m, 300), replace=False)
+        l = b[f"{kind}_node_lower"][:m].cpu().numpy()[idx]
+        u = b[f"{kind}_node_upper"][:m].cpu().numpy()[idx]
+        assert np.all(o64.classify_box(l, u) == code), f"{name}: a BF16 {kind} leaf is not {kind} under FP64"
+        checked += len(idx)
+    note(f"  FP64-checked determinate leaves: {checked}")
+    if name != "c4_relu_512x8":
+        assert checked > 0, f"{name}: no determinate leaves to check"
 
 
 # ------------------------------------------------------------ closest point
 # sizes the oracle's batch stack finishes in seconds (the reference search on a
 # deep trained model such as fox expands millions of nodes)
-@pytest.mark.parametrize("name,nq,eps", [("c1_relu_3x32", 300, 0.01), ("small_tanh", 64, 0.05)])
-def test_closest_point_matches_oracle(name, nq, eps):
+@pytest.mark.parametrize("name,nq,eps,batch", [("c1_relu_3x32", 300, 0.01, 1 << 20), ("small_tanh", 64, 0.05, 1 << 20),
+                                              ("c1_relu_3x32", 300, 0.01, 256)])
+def test_closest_point_matches_oracle(name, nq, eps, batch):
+    """closest_point against the oracle's batch-stack restatement.  With one
+    batch for the whole stack no query depends on another's decisions: every
+    differing distance must be near-threshold.  With batches of 256 the
+    queries share the stack (a near decision of one changes the rounds of the
+    others), so queries processed after the first near decision are exempt
+    too, and every distance stays within a few eps."""
     from oracle import closest as ocp
-    note(f"[closest point] {name}")
+    note(f"[closest point] {name} batch {batch}")
     p = model(name)
     func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
     q = np.random.default_rng(3).uniform(-1, 1, size=(nq, 3)).astype(np.float32)
     d, loc = kd_tree.closest_point(func, p, LO, UP, torch.as_tensor(q, device="cuda"), eps=eps,
-                                   batch_process_size=256)
-    rd, rloc, info = ocp.closest_point(ofuncs.OracleFunc(p, "affine_fixed"), LO, UP, q, eps=eps,
-                                       batch_process_size=256)
+                                   batch_process_size=batch)
+    rd, rloc, info = ocp.closest_point(margins.TwinFunc(p, "affine_fixed"), LO, UP, q, eps=eps,
+                                       batch_process_size=batch, track_near=True)
     d, loc = d.cpu().numpy(), loc.cpu().numpy()
     same = (d == rd)
-    note(f"  {same.mean() * 100:.2f}% distances identical; gpu processed "
-         f"{kd_tree.closest_point.last_n_processed} nodes, oracle {info['n_processed']}")
-    assert same.mean() >= 0.98
+    exempt = info["near"]
+    if batch < info["n_processed"] and info["near_round"] is not None:
+        exempt = exempt | (info["last_round"] >= info["near_round"])
+    only_near(same, exempt, "distances",
+              f"gpu processed {kd_tree.closest_point.last_n_processed} nodes, oracle {info['n_processed']}, "
+              f"{info['rounds']} rounds")
+    assert np.all(np.abs(d - rd)[np.isfinite(rd)] <= 5 * eps)
     fin = np.isfinite(rd) & same
     np.testing.assert_array_equal(loc[fin], rloc[fin])
 
@@ -983,17 +1032,28 @@ def test_truncate_append_match_oracle(mode, kw, name):
     codes, bounds = func.classify_box(p, torch.as_tensor(lo, device="cuda"),
                                       torch.as_tensor(up, device="cuda"), return_bounds=True)
     codes, bounds = codes.cpu().numpy(), bounds.cpu().numpy()
-    r_lo, r_hi = ofuncs.OracleFunc(p, mode, np.float32, **kw).bounds_box(lo, up)
+    gaps = []
+    oaff.log_selection_gaps(gaps)
+    try:
+        r_lo, r_hi = ofuncs.OracleFunc(p, mode, np.float32, **kw).bounds_box(lo, up)
+    finally:
+        oaff.log_selection_gaps(None)
+    gap = np.min(np.stack(gaps), axis=0)
     d_lo, d_hi = ofuncs.OracleFunc(p, mode, np.float64, **kw).bounds_box(lo, up)
     noise_lo, noise_hi = np.abs(r_lo - d_lo), np.abs(r_hi - d_hi)
     e_lo, e_hi = np.abs(bounds[:, 0] - r_lo), np.abs(bounds[:, 1] - r_hi)
     scale = np.abs(d_lo) + np.abs(d_hi)
-    # row selection can flip on near-ties between rows; such cells are counted
-    bad = (e_lo > 1e-5 * np.max(np.abs(r_lo)) + 2 * noise_lo) | (e_hi > 1e-5 * np.max(np.abs(r_hi)) + 2 * noise_hi)
-    note(f"  max rel bound dev {np.max(np.maximum(e_lo, e_hi) / scale):.2e}, cells off the bar {bad.sum()}")
-    assert bad.mean() <= 0.01
+    # the rows kept are chosen by comparing FP32 magnitudes: a cell whose last
+    # kept and first dropped row differ by less than TIE_REL (relative) may
+    # keep another row on the GPU, a near tie
+    tie = gap <= TIE_REL
+    off = (e_lo > 1e-5 * np.max(np.abs(r_lo)) + 2 * noise_lo) | (e_hi > 1e-5 * np.max(np.abs(r_hi)) + 2 * noise_hi)
     ref_codes = oaff.sign_code(r_lo, r_hi, 0.0)
-    assert np.mean(codes != ref_codes) <= 0.01
+    near = (np.abs(d_lo) <= NEAR_REL * scale + 4 * noise_lo) | (np.abs(d_hi) <= NEAR_REL * scale + 4 * noise_hi)
+    note(f"  max rel bound dev {np.max(np.maximum(e_lo, e_hi) / scale):.2e}; cells off the bound bar {off.sum()}, "
+         f"near-tie cells {tie.sum()} (smallest selection gap {gap.min():.2e})")
+    assert not np.any(off & ~tie), np.sort(gap[off])[:5]
+    only_near(codes == ref_codes, near | tie, "codes")
 
 
 # ------------------------------------------------------------ kernel variants

@@ -2,8 +2,9 @@
 
 Bar: identical outputs, bit for bit.  A served request runs the same kernels
 on the same inputs as the standalone call: rays are independent of one
-another (a merged cast gives every ray what a lone cast gives it), and each
-tree build and closest-point batch runs as one call of its own.  Checked with
+another (a merged cast gives every ray what a lone cast gives it, each
+request being its own group of rays with its own models and options), and
+each tree build and closest-point batch runs as one call of its own.  Checked with
 requests merged (one worker kept busy so that ray requests queue), with three
 workers overlapping on the device, with tree builds replayed from a captured
 hipGraph over different root boxes, with memory admission serialising
@@ -80,6 +81,42 @@ def test_merged_rays_equal_lone_casts():
     assert max(batched) > 1
     for a, g in zip(alone, got):
         assert same_rays(a, g)
+        assert a[3] == g[3]          # each request's own N_evals
+
+
+def test_rays_of_different_models_and_options_share_a_cast():
+    """Ray requests of different models (same width template and config) and
+    different cast options are merged into one persistent cast, each as its
+    own ray group: every request's (t, hit_id, count, N_evals) equals its lone
+    cast's, bit for bit."""
+    note("[serve] one cast over requests of different models and options")
+    p1 = models.random_mlp([24, 24], activation="relu", seed=31)
+    p2 = models.random_mlp([40, 32, 16], activation="tanh", seed=32)
+    p3 = models.config_model("c1_relu_3x32")
+    fs = []
+    for p in (p1, p2, p3):
+        f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+        fs.append((f, p))
+    o1 = queries.get_default_cast_opts()
+    o1["max_dist"] = 6.0
+    o2 = dict(o1, hit_eps=0.002, n_max_step=200, interval_grow_fac=1.3)
+    roots, dirs = camera(64)
+    chunks = tiles(roots, dirs, [900, 1500, 400, 1296])
+    plan = [((fs[0],), o1, chunks[0]), ((fs[1],), o2, chunks[1]), ((fs[2], fs[0]), o1, chunks[2]),
+            ((fs[1],), o1, chunks[3])]
+    alone = [queries.cast_rays(tuple(f for f, _ in fp), tuple(p for _, p in fp), r, d, o) for fp, o, (r, d) in plan]
+    (fb, pb), _, _ = setup()
+    with serve.Server(n_streams=1) as srv:
+        busy = srv.submit_tree(fb, pb, LO, UP, split_depth=16)
+        reqs = [srv.submit_rays(tuple(f for f, _ in fp), tuple(p for _, p in fp), r, d, o) for fp, o, (r, d) in plan]
+        busy.result()
+        got = [r.result() for r in reqs]
+        batched = [r.info["n_batched"] for r in reqs]
+    note(f"  merged group sizes {batched}")
+    assert max(batched) >= 3
+    for i, (a, g) in enumerate(zip(alone, got)):
+        assert same_rays(a, g), f"request {i}"
+        assert a[3] == g[3], f"request {i}: N_evals {g[3]} alone {a[3]}"
 
 
 def test_mix_equals_standalone():
@@ -187,3 +224,35 @@ def test_other_modes_equal_lone_calls():
     assert same_tree(alone_all, ra) and same_tree(alone_slope, rb)
     assert same_rays(alone_rays, rc)
     assert torch.equal(rd[0], alone_cp[0]) and torch.equal(rd[1], alone_cp[1])
+
+
+def test_graph_not_replayed_for_a_new_model_at_a_freed_address():
+    """A cached tree graph holds its model's device weights in its captured
+    arguments; a model created after another is destroyed may get the same
+    handle address, and must not be served by the old model's graph."""
+    import gc
+    from niq import mlp as nmlp
+    note("[serve] graph cache after a model is destroyed and another created")
+    with serve.Server(n_streams=1) as srv:
+        pa = models.random_mlp([40, 40], activation="relu", seed=21)
+        fa, _ = implicit_mlp_utils.implicit_from_params(pa, "affine_fixed")
+        first = srv.submit_tree(fa, pa, LO, UP, split_depth=9)
+        first.result()
+        again = srv.submit_tree(fa, pa, LO, UP, split_depth=9)
+        again.result()
+        assert not first.info["graph_reused"] and again.info["graph_reused"]   # the same model reuses its graph
+        addr_a = nmlp.device_model(pa).handle.value
+        # destroy model a: drop every reference, then create b of the same shape
+        srv._models.clear()
+        nmlp.forget_device_model(pa)
+        del first, again, fa, pa
+        gc.collect()
+        pb = models.random_mlp([40, 40], activation="relu", seed=22)
+        fb, _ = implicit_mlp_utils.implicit_from_params(pb, "affine_fixed")
+        addr_b = nmlp.device_model(pb).handle.value
+        note(f"  handle of a {addr_a:#x}, of b {addr_b:#x}")
+        alone = kd_tree.construct_uniform_unknown_levelset_tree(fb, pb, LO, UP, split_depth=9)
+        got = srv.submit_tree(fb, pb, LO, UP, split_depth=9)
+        res = got.result()
+        assert not got.info["graph_reused"]
+    assert same_tree(alone, res)
