@@ -200,27 +200,21 @@ def bf16_bound_quality(name, n_cells=512, seed=4):
             "determinate_bf16": int((c16 != 0).sum()), "determinate_fp64": int((c64 != 0).sum())}
 
 
-SERVE_SHAPES = (("c1 3->32x3->1 relu", [32] * 3, "relu", "he"),
-                ("c2 3->256x6->1 relu", [256] * 6, "relu", "he"),
-                ("c3 3->128x8->1 tanh", [128] * 8, "tanh", "xavier"))
-
-
-def serving_trace(n_req, rate, seed=0):
-    """A reduced serving trace: eight resident MLPs (seeds 10-17, shapes drawn
-    from the C1-C3 shapes) under affine_fixed; Poisson arrivals (seed 0) of
-    50 % ray-cast tiles (side U{64..256} of the 60-degree camera at distance 3),
-    30 % closest-point batches (U{1..64} points in [-1,1]^3, eps 0.05) and
-    20 % level-set trees (split depth U{12..21}); memory cap 200 GB.  The tile
-    sides and closest-point batches are scaled down from 64-512 px and
-    1-65,536 points so that the trace runs in about a minute.  Reports p50/p99
-    latency per kind and overall, and requests/s."""
+def serving_trace(n_req, rate, seed=0, max_side=512, max_points=65536):
+    """The serving configuration: eight resident MLPs (seeds 10-17, shapes
+    drawn from the three single-GPU models) under affine_fixed; Poisson
+    arrivals (seed 0) of 50 % ray-cast tiles (side U{64..max_side} of the
+    60-degree camera at distance 3), 30 % closest-point batches
+    (U{1..max_points} points in [-1,1]^3, eps 0.05) and 20 % level-set trees
+    (split depth U{12..21}); memory cap 200 GB.  n_req requests (the
+    configuration asks for 10,000) at `rate` requests/s.  Reports p50/p99
+    latency per kind and overall, requests/s and how many ray requests were
+    merged into shared casts."""
     import torch
     from niq import models, implicit_mlp_utils, queries, render, serve
-    rng = np.random.default_rng(seed)
+    resident_models, rng = models.serving_models(seed)
     resident = []
-    for s_ in range(10, 18):
-        name, hidden, act, init = SERVE_SHAPES[int(rng.integers(0, 3))]
-        p = models.random_mlp(hidden, activation=act, init=init, seed=s_)
+    for name, p in resident_models:
         f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
         resident.append((name, f, p))
     eye = np.array([0., 0., 3.], np.float32)
@@ -235,10 +229,9 @@ def serving_trace(n_req, rate, seed=0):
         u = rng.uniform()
         k = int(rng.integers(0, len(resident)))
         if u < 0.5:
-            side = int(rng.integers(64, 257))
-            plan.append(("rays", k, side))
+            plan.append(("rays", k, int(rng.integers(64, max_side + 1))))
         elif u < 0.8:
-            plan.append(("closest_point", k, int(rng.integers(1, 65))))
+            plan.append(("closest_point", k, int(rng.integers(1, max_points + 1))))
         else:
             plan.append(("tree", k, int(rng.integers(12, 22))))
     # inputs built before the clock starts (resident in HBM)
@@ -267,21 +260,22 @@ def serving_trace(n_req, rate, seed=0):
                 reqs.append(srv.submit_closest_point(f, p, lo, up, inp, eps=0.05))
             else:
                 reqs.append(srv.submit_tree(f, p, lo, up, split_depth=arg))
-            if len(reqs) % 50 == 0:
+            if len(reqs) % 25 == 0:
                 progress(f"serving: {len(reqs)}/{len(plan)} requests submitted")
         for i, r in enumerate(reqs):
             r.result()
-            if (i + 1) % 20 == 0:
+            if (i + 1) % 25 == 0:
                 progress(f"serving: {i + 1}/{len(reqs)} requests done")
         wall = time.perf_counter() - t0
     lat = serve.latency_summary(reqs)
     merged = sum(1 for r in reqs if r.kind == "rays" and r.info["n_batched"] > 1)
     replays = sum(1 for r in reqs if r.kind == "tree" and r.info["graph_replay"])
-    return {"workload": f"{n_req} requests, Poisson {rate:g}/s (seed {seed}), 8 resident MLPs, "
-                        "50% ray tiles 64-256 px, 30% closest-point 1-64 pts (eps 0.05), 20% trees depth 12-21",
+    reused = sum(1 for r in reqs if r.kind == "tree" and r.info["graph_reused"])
+    return {"workload": f"{n_req} requests, Poisson {rate:g}/s (seed {seed}), 8 resident MLPs, 50% ray tiles "
+                        f"64-{max_side} px, 30% closest-point 1-{max_points} pts (eps 0.05), 20% trees depth 12-21",
             "requests_per_s": n_req / wall, "wall_s": wall, "latency_ms": lat,
             "rays_merged_requests": merged, "arrival_rate_per_s": rate, "trees_from_graphs": replays,
-            "models": [m[0] for m in resident]}
+            "trees_from_cached_graphs": reused, "models": [m[0] for m in resident]}
 
 
 def next_rows():
@@ -378,25 +372,23 @@ def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):
     another and then all submitted at once to the serving loop (4 streams)."""
     import torch
     from niq import models, implicit_mlp_utils, kd_tree, serve, autotune
-    rng = np.random.default_rng(seed)
     fams = []
-    for i in range(n_models):
-        w = int(rng.choice([32, 64, 128, 256]))
-        nl = int(rng.integers(3, 9))
-        p = models.random_mlp([w] * nl, activation="tanh", init="xavier", seed=20 + i)
+    tuned = []
+    for w, nl, p in models.concurrent_tree_models(n_models, seed):
         f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
-        autotune.tune(f, p, split_depth=depth, n_boxes=1 << 14, reps=2)
+        tuned.append(autotune.tune(f, p, split_depth=depth, n_boxes=1 << 14, reps=2))
         fams.append((f, p, w, nl))
     lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
     for f, p, _, _ in fams:
         kd_tree.build_tree(f, p, lo, up, split_depth=8)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    boxes = 0
+    boxes, flops = 0, 0
     for i in range(n_trees):
-        f, p, _, _ = fams[i % n_models]
+        f, p, w, nl = fams[i % n_models]
         _, st = kd_tree.build_tree(f, p, lo, up, split_depth=depth)
         boxes += st["n_classified"]
+        flops += st["n_classified"] * models.flops_per_box([3] + [w] * nl + [1], rows=5)
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
     with serve.Server(n_streams=4) as srv:
@@ -416,8 +408,15 @@ def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):
                         "(widths 32-256, 3-8 hidden layers), affine_fixed",
             "boxes": boxes, "sequential_s": t_seq, "served_s": t_srv,
             "boxes_per_s_sequential": boxes / t_seq, "boxes_per_s_served": boxes / t_srv,
+            "speedup_served_over_sequential": t_seq / t_srv,
+            # the trees' contraction work (5 rows through every layer per box)
+            # over each wall time: how close the builds already are to the
+            # FP32 MFMA roof, which bounds what overlapping them can gain
+            "tflops_sequential": flops / t_seq / 1e12, "tflops_served": flops / t_srv / 1e12,
+            "frac_fp32_mfma_served": flops / t_srv / 1e12 / PEAK_FP32_TFLOPS,
             "latency_ms": lat.get("tree"),
             "variants": [int(f.cfg.variant) for f, _, _, _ in fams],
+            "tiles": [r.get("shape") for r in tuned],
             "shapes": [f"{w}x{nl}" for _, _, w, nl in fams]}
 
 

@@ -65,9 +65,11 @@ typedef struct niq_eval_cfg {
   int truncate_count;  /* affine_truncate: rows kept */
   int n_append;        /* affine_append: rows appended per activation */
   float recheck_tol;   /* > 0: FP64 re-test of cells within tol*scale of the threshold */
-  int variant;         /* kernel variant, 0 = default; precision mode: 1 streams
-                          the packed bf16 weight copies, 2 splits FP32 weights
-                          in registers (the default) */
+  int variant;         /* kernel variant, 0 = default; plain FP32: 1 TileCfg,
+                          2 TileCfgAlt, 3 TileCfg with 256 registers
+                          (niq_tile_shape); precision mode: 1 streams the
+                          packed bf16 weight copies, 2 splits FP32 weights in
+                          registers (the default), 3 layer by layer */
   float sdf_lipschitz; /* NIQ_MODE_SDF: Lipschitz bound of f (sdf_lipschitz kwarg) */
   int ray_variant;     /* cast_rays tile shape, 0 = default; 1 and 2 are the
                           alternatives the autotuner times (same results) */
@@ -103,6 +105,13 @@ niq_status niq_classify_general_boxes(const niq_model* m, const niq_eval_cfg* cf
                                       const float* center, const float* vecs, int64_t n,
                                       int v, int d, float offset, int32_t* codes,
                                       float* bounds, void* stream);
+
+/* Tile shape of a kernel variant, for autotuners and reports: which 0 is the
+ * plain FP32 classify pass (eval cfg variant 0-3), 1 the ray caster
+ * (ray_variant 0-2); width a layer width of the model (its template width is
+ * used).  *nb: boxes or ray slots per workgroup, *nw: waves per workgroup,
+ * *occ: waves per SIMD the register budget is set for.  Host only. */
+niq_status niq_tile_shape(int which, int variant, int width, int* nb, int* nw, int* occ);
 
 /* ---- plain evaluation ----------------------------------------------------
  * Replaces ImplicitFunction.__call__ (src/affine.py:26-28 -> src/mlp.py:96-113,
@@ -294,6 +303,15 @@ niq_status niq_bulk_sums_nodes(const niq_model* m, const float* unknown_lower, c
 
 /* ---- normals and shading -------------------------------------------------------
  * The image half of render_image (src/render.py:94-150). */
+
+/* Replaces generate_camera_rays (src/render.py:26-50): the res x res pinhole
+ * rays, pixel p = row * res + col, in the reference's float32 operations.
+ * look, up, left: the camera frame as generate_camera_rays forms it (up made
+ * orthogonal to look and normalised, left = look x up); tan_half =
+ * tan(radians(fov) / 2) in float32.  out_roots, out_dirs: (res^2, 3) float32
+ * device buffers.  Asynchronous on `stream`. */
+niq_status niq_camera_rays(const float eye[3], const float look[3], const float up[3], const float left[3],
+                           int res, float tan_half, float* out_roots, float* out_dirs, void* stream);
 
 /* hit_pos = roots + t * dirs (src/render.py:139), float32, unfused; (n, 3). */
 niq_status niq_hit_positions(const float* roots, const float* dirs, const float* t, int64_t n,

@@ -8,6 +8,7 @@ namespace {
 template <int KIND, int V, int W, class T = TileCfg<W>>
 niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
   constexpr int NB = T::NB, NW = T::NW;
+  constexpr int OCC = T::OCC ? std::min(T::OCC, LdsOcc<Planes<KIND, V, 0>::R, NB, NW, W>::value) : 0;
   constexpr int R = Planes<KIND, V, 0>::R;
   if constexpr (!Fits<R, NB, W>::value) {
     // the planes live in global memory instead (stream-ordered scratch, one
@@ -30,7 +31,7 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
     }
     return NIQ_OK;
   } else {
-    auto k = k_classify<KIND, V, NB, NW, W>;
+    auto k = k_classify<KIND, V, NB, NW, W, false, OCC>;
     const long long tiles = (max_n + NB - 1) / NB;
     if (tiles == 0) return NIQ_OK;
     const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
@@ -46,8 +47,9 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
   }
 }
 
-// Tile variants: variant 2 selects TileCfgAlt for the forms trees use most
-// (interval and affine_fixed over 3 box vectors); otherwise TileCfg.
+// Tile variants for the forms trees use most (interval and affine_fixed over
+// 3 box vectors): 1 (and 0) TileCfg, 2 TileCfgAlt, 3 TileCfgRegs; the other
+// forms always take TileCfg.
 template <int KIND, int V>
 niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s,
                            int variant = 0) {
@@ -70,6 +72,15 @@ niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long 
         case 128: return launch_classify_w<KIND, V, 128, TileCfgAlt<128>>(m, a, max_n, s);
         case 256: return launch_classify_w<KIND, V, 256, TileCfgAlt<256>>(m, a, max_n, s);
         case 512: return launch_classify_w<KIND, V, 512, TileCfgAlt<512>>(m, a, max_n, s);
+      }
+      return fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+    if (variant == 3) {
+      switch (m->wt) {
+        case 64: return launch_classify_w<KIND, V, 64, TileCfgRegs<64>>(m, a, max_n, s);
+        case 128: return launch_classify_w<KIND, V, 128, TileCfgRegs<128>>(m, a, max_n, s);
+        case 256: return launch_classify_w<KIND, V, 256, TileCfgRegs<256>>(m, a, max_n, s);
+        case 512: return launch_classify_w<KIND, V, 512, TileCfgRegs<512>>(m, a, max_n, s);
       }
       return fail(NIQ_ERR_UNSUPPORTED, "width");
     }
@@ -194,7 +205,45 @@ niq_status classify_with_recheck(const niq_model* m, const niq_eval_cfg* cfg, in
 }
 }  // namespace
 
+namespace {
+template <class T, int W>
+void shape_of(int R, int* nb, int* nw, int* occ) {
+  *nb = T::NB; *nw = T::NW;
+  const int wgs = (int)((size_t)kLdsLimit / ((size_t)R * T::NB * W * 4));
+  const int lds = std::max(1, std::min(4, wgs * T::NW / 4));
+  *occ = T::OCC ? std::min(T::OCC, lds) : lds;
+}
+template <int W>
+niq_status tile_shape_w(int which, int variant, int* nb, int* nw, int* occ) {
+  if (which == 0) {   // plain FP32 classify, affine_fixed over 3 box vectors (5 planes)
+    switch (variant) {
+      case 0: case 1: shape_of<TileCfg<W>, W>(5, nb, nw, occ); return NIQ_OK;
+      case 2: shape_of<TileCfgAlt<W>, W>(5, nb, nw, occ); return NIQ_OK;
+      case 3: shape_of<TileCfgRegs<W>, W>(5, nb, nw, occ); return NIQ_OK;
+    }
+  } else if (which == 1) {   // ray caster, affine_fixed (5 planes)
+    switch (variant) {
+      case 0: shape_of<RayTile<W, 0>, W>(5, nb, nw, occ); return NIQ_OK;
+      case 1: shape_of<RayTile<W, 1>, W>(5, nb, nw, occ); return NIQ_OK;
+      case 2: shape_of<RayTile<W, 2>, W>(5, nb, nw, occ); return NIQ_OK;
+    }
+  }
+  return fail(NIQ_ERR_INVALID, "unknown kernel or variant");
+}
+}  // namespace
+
 extern "C" {
+
+niq_status niq_tile_shape(int which, int variant, int width, int* nb, int* nw, int* occ) {
+  if (!nb || !nw || !occ) return fail(NIQ_ERR_INVALID, "null output");
+  switch (template_width(width)) {
+    case 64: return tile_shape_w<64>(which, variant, nb, nw, occ);
+    case 128: return tile_shape_w<128>(which, variant, nb, nw, occ);
+    case 256: return tile_shape_w<256>(which, variant, nb, nw, occ);
+    case 512: return tile_shape_w<512>(which, variant, nb, nw, occ);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width above 512");
+}
 
 niq_status niq_classify_boxes(const niq_model* m, const niq_eval_cfg* cfg, const float* lower,
                               const float* upper, int64_t n, int d, float offset, int32_t* codes,

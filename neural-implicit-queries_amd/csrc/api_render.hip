@@ -41,6 +41,51 @@ niq_status launch_grad_normals(const niq_model* m, const int* list, const int* c
 
 }  // namespace
 
+namespace {
+
+// One pinhole camera ray per pixel (src/render.py:26-50), in the reference's
+// float32 operation order: x, y from linspace(-1, 1, res) (formed in double,
+// the last sample exactly 1, then rounded), pixel p = row * res + col ('xy'
+// meshgrid), pos = (look + left (x t)) + up (y t), dir = pos / |pos| with
+// |pos| = sqrt((x0^2 + x1^2) + x2^2) correctly rounded.
+__global__ void k_camera_rays(float3 eye, float3 look, float3 up, float3 left, int res, float tan_half,
+                              float* __restrict__ roots, float* __restrict__ dirs) {
+  const long long n = (long long)res * res;
+  const double step = res > 1 ? 2.0 / (double)(res - 1) : 0.0;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+    const int row = (int)(p / res), col = (int)(p % res);
+    const float cx = col == res - 1 && res > 1 ? 1.f : (float)((double)col * step + -1.0);
+    const float cy = row == res - 1 && res > 1 ? 1.f : (float)((double)row * step + -1.0);
+    const float sx = __fmul_rn(cx, tan_half), sy = __fmul_rn(cy, tan_half);
+    float q[3];
+    const float lk[3] = {look.x, look.y, look.z}, lf[3] = {left.x, left.y, left.z}, u[3] = {up.x, up.y, up.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q[k] = __fadd_rn(__fadd_rn(lk[k], __fmul_rn(lf[k], sx)), __fmul_rn(u[k], sy));
+    // sqrtf: the correctly rounded expansion (HIP compiles it so by default;
+    // __fsqrt_rn is a bare v_sqrt_f32, within 1 ulp)
+    const float nrm = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(q[0], q[0]), __fmul_rn(q[1], q[1])), __fmul_rn(q[2], q[2])));
+    roots[3 * p] = eye.x; roots[3 * p + 1] = eye.y; roots[3 * p + 2] = eye.z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dirs[3 * p + k] = __fdiv_rn(q[k], nrm);
+  }
+}
+
+}  // namespace
+
+extern "C" niq_status niq_camera_rays(const float eye[3], const float look[3], const float up[3], const float left[3],
+                                      int res, float tan_half, float* out_roots, float* out_dirs, void* stream) {
+  if (!eye || !look || !up || !left) return fail(NIQ_ERR_INVALID, "null camera vector");
+  if (res < 1 || res > 65536) return fail(NIQ_ERR_INVALID, "resolution out of range");
+  if (!out_roots || !out_dirs) return fail(NIQ_ERR_INVALID, "bad buffers");
+  const long long n = (long long)res * res;
+  const unsigned g = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_camera_rays, dim3(g), dim3(256), 0, (hipStream_t)stream, make_float3(eye[0], eye[1], eye[2]),
+                     make_float3(look[0], look[1], look[2]), make_float3(up[0], up[1], up[2]),
+                     make_float3(left[0], left[1], left[2]), res, tan_half, out_roots, out_dirs);
+  HIPCHK(hipGetLastError());
+  return NIQ_OK;
+}
+
 extern "C" niq_status niq_hit_positions(const float* roots, const float* dirs, const float* t, int64_t n,
                                         float* out, void* stream) {
   if (n < 0 || (n > 0 && (!roots || !dirs || !t || !out))) return fail(NIQ_ERR_INVALID, "bad buffers");

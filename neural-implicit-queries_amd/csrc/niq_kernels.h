@@ -6,28 +6,35 @@
 #include "niq_device.h"
 
 // Workgroup shapes per padded max width: boxes per tile (NB) and waves (NW).
+// OCC: waves per SIMD the register budget is set for (0: what LDS admits).
 template <int WMAX> struct TileCfg;
-template <> struct TileCfg<64>  { static constexpr int NB = 32, NW = 4, NBP = 64; };
-template <> struct TileCfg<128> { static constexpr int NB = 32, NW = 4, NBP = 64; };
-template <> struct TileCfg<256> { static constexpr int NB = 16, NW = 8, NBP = 64; };
-template <> struct TileCfg<512> { static constexpr int NB = 16, NW = 8, NBP = 64; };
+template <> struct TileCfg<64>  { static constexpr int NB = 32, NW = 4, NBP = 64, OCC = 0; };
+template <> struct TileCfg<128> { static constexpr int NB = 32, NW = 4, NBP = 64, OCC = 0; };
+template <> struct TileCfg<256> { static constexpr int NB = 16, NW = 8, NBP = 64, OCC = 0; };
+template <> struct TileCfg<512> { static constexpr int NB = 16, NW = 8, NBP = 64, OCC = 0; };
 
 // Alternative tiles offered to the autotuner (eval cfg variant 2): twice the
 // boxes per workgroup where LDS allows it (weights read half as often per box),
 // fewer waves per workgroup at width 512.
 template <int WMAX> struct TileCfgAlt;
-template <> struct TileCfgAlt<64>  { static constexpr int NB = 64, NW = 4; };
-template <> struct TileCfgAlt<128> { static constexpr int NB = 64, NW = 8; };
-template <> struct TileCfgAlt<256> { static constexpr int NB = 32, NW = 8; };
-template <> struct TileCfgAlt<512> { static constexpr int NB = 16, NW = 4; };
+template <> struct TileCfgAlt<64>  { static constexpr int NB = 64, NW = 4, OCC = 0; };
+template <> struct TileCfgAlt<128> { static constexpr int NB = 64, NW = 8, OCC = 0; };
+template <> struct TileCfgAlt<256> { static constexpr int NB = 32, NW = 8, OCC = 0; };
+template <> struct TileCfgAlt<512> { static constexpr int NB = 16, NW = 4, OCC = 0; };
+
+// The default tile with a 256-register budget (eval cfg variant 3): no
+// spills, fewer resident workgroups where LDS would admit more.
+template <int WMAX> struct TileCfgRegs {
+  static constexpr int NB = TileCfg<WMAX>::NB, NW = TileCfg<WMAX>::NW, OCC = 2;
+};
 
 // Tiles for levels too small to fill the chip: a workgroup's latency bounds
 // them, so its feature tiles are spread over more waves (one or two each).
 template <int WMAX> struct TileCfgWide;
-template <> struct TileCfgWide<64>  { static constexpr int NB = 16, NW = 4; };
-template <> struct TileCfgWide<128> { static constexpr int NB = 16, NW = 8; };
-template <> struct TileCfgWide<256> { static constexpr int NB = 16, NW = 16; };
-template <> struct TileCfgWide<512> { static constexpr int NB = 16, NW = 16; };
+template <> struct TileCfgWide<64>  { static constexpr int NB = 16, NW = 4, OCC = 0; };
+template <> struct TileCfgWide<128> { static constexpr int NB = 16, NW = 8, OCC = 0; };
+template <> struct TileCfgWide<256> { static constexpr int NB = 16, NW = 16, OCC = 0; };
+template <> struct TileCfgWide<512> { static constexpr int NB = 16, NW = 16, OCC = 0; };
 
 constexpr int kLdsLimit = 160 * 1024;
 
@@ -128,8 +135,8 @@ __device__ __forceinline__ void fill_box_inputs(float* X, const ClassifyArgs& a,
   }
 }
 
-template <int KIND, int V, int NB, int NW, int WMAX, bool GX = false>
-__global__ void __launch_bounds__(NW * 64, (GX ? 2 : LdsOcc<Planes<KIND, V, 0>::R, NB, NW, WMAX>::value))
+template <int KIND, int V, int NB, int NW, int WMAX, bool GX = false, int OCC = 0>
+__global__ void __launch_bounds__(NW * 64, (OCC ? OCC : (GX ? 2 : LdsOcc<Planes<KIND, V, 0>::R, NB, NW, WMAX>::value)))
 k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
   using PL = Planes<KIND, V, 0>;
   constexpr int NT = NW * 64;
@@ -597,15 +604,15 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
     const unsigned long long dead = __ballot(slot_lane && rid < 0);
     const int n_dead = __popcll(dead);
     __syncthreads();
-    if (threadIdx.x == 0)
-      *s_base = (n_dead && !exhausted) ? (long long)atomicAdd(G.next_ray, (unsigned long long)n_dead) : G.n;
+    if (threadIdx.x == 0)   // -1: no claim (no dead slot, or nothing left to claim)
+      *s_base = (n_dead && !exhausted) ? (long long)atomicAdd(G.next_ray, (unsigned long long)n_dead) : -1LL;
     __syncthreads();
     const long long base = *s_base;
-    if (base + n_dead >= G.n) exhausted = true;
-    if (slot_lane && rid < 0) {
+    if (base >= 0 && base + n_dead >= G.n) exhausted = true;
+    if (base >= 0 && slot_lane && rid < 0) {
       const int rank = __popcll(dead & ((1ull << lane) - 1ull));
       const long long r = base + rank;
-      if (r >= 0 && r < G.n) {
+      if (r < G.n) {
         rid = (int)r;
         t = 0.f; step = G.init_size; cnt = 0; iters = 0;
         for (int k = 0; k < 3; ++k) { ro[k] = G.roots[3 * r + k]; di[k] = G.dirs[3 * r + k]; }
@@ -613,7 +620,8 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
     }
     const bool any_live = __ballot(slot_lane && rid >= 0) != 0ull;
     if (!any_live) {   // uniform: every wave holds the same slot state
-      // drained and the group has no rays left: on to the next group
+      // every slot was dead and the claim found no ray left (or none was
+      // made): the group is drained here, on to the next group
       if (++seen >= a.n_groups) break;
       gi = gi + 1 == a.n_groups ? 0 : gi + 1;
       G = a.groups[gi];

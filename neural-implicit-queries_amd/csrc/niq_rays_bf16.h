@@ -42,15 +42,15 @@ k_cast_rays_bf16(const DevModelBf16* __restrict__ models, RayArgs a) {
     const unsigned long long dead = __ballot(slot_lane && rid < 0);
     const int n_dead = __popcll(dead);
     __syncthreads();
-    if (threadIdx.x == 0)
-      *s_base = (n_dead && !exhausted) ? (long long)atomicAdd(G.next_ray, (unsigned long long)n_dead) : G.n;
+    if (threadIdx.x == 0)   // -1: no claim (no dead slot, or nothing left to claim)
+      *s_base = (n_dead && !exhausted) ? (long long)atomicAdd(G.next_ray, (unsigned long long)n_dead) : -1LL;
     __syncthreads();
     const long long base = *s_base;
-    if (base + n_dead >= G.n) exhausted = true;
-    if (slot_lane && rid < 0) {
+    if (base >= 0 && base + n_dead >= G.n) exhausted = true;
+    if (base >= 0 && slot_lane && rid < 0) {
       const int rank = __popcll(dead & ((1ull << lane) - 1ull));
       const long long r = base + rank;
-      if (r >= 0 && r < G.n) {
+      if (r < G.n) {
         rid = (int)r;
         t = 0.f; step = G.init_size; cnt = 0; iters = 0;
         for (int k = 0; k < 3; ++k) { ro[k] = G.roots[3 * r + k]; di[k] = G.dirs[3 * r + k]; }
@@ -58,7 +58,8 @@ k_cast_rays_bf16(const DevModelBf16* __restrict__ models, RayArgs a) {
     }
     const bool any_live = __ballot(slot_lane && rid >= 0) != 0ull;
     if (!any_live) {   // uniform: every wave holds the same slot state
-      // drained and the group has no rays left: on to the next group
+      // every slot was dead and the claim found no ray left (or none was
+      // made): the group is drained here, on to the next group
       if (++seen >= a.n_groups) break;
       gi = gi + 1 == a.n_groups ? 0 : gi + 1;
       G = a.groups[gi];

@@ -24,12 +24,12 @@ MODE = {"interval": 0, "affine_fixed": 1, "affine_truncate": 2, "affine_append":
 PREC = {"fp32": 0, "bf16": 1}
 
 EXPORTS = ("niq_model_create", "niq_model_destroy", "niq_model_device_bytes",
-           "niq_classify_boxes", "niq_classify_general_boxes", "niq_eval_points",
+           "niq_classify_boxes", "niq_classify_general_boxes", "niq_tile_shape", "niq_eval_points",
            "niq_tree_workspace_bytes", "niq_tree_build", "niq_tree_enqueue",
            "niq_tree_counters_bytes", "niq_tree_decode", "niq_cast_workspace_bytes",
            "niq_cast_rays", "niq_cast_rays_frustum", "niq_find_any_intersection", "niq_closest_point",
            "niq_sample_workspace_bytes", "niq_sample_surface_nodes", "niq_bulk_sums_nodes",
-           "niq_hit_positions", "niq_outward_normals", "niq_shade_image", "niq_mc_extract", "niq_mc_table",
+           "niq_camera_rays", "niq_hit_positions", "niq_outward_normals", "niq_shade_image", "niq_mc_extract", "niq_mc_table",
            "niq_serve_create", "niq_serve_destroy", "niq_serve_submit_rays", "niq_serve_submit_tree",
            "niq_serve_submit_closest_point", "niq_serve_poll", "niq_serve_bytes_in_use",
            "niq_last_error", "niq_version")
@@ -129,6 +129,7 @@ def load_library():
         "niq_model_device_bytes": (S, [P]),
         "niq_classify_boxes": (I, [P, P, P, P, I64, I, F, P, P, P]),
         "niq_classify_general_boxes": (I, [P, P, P, P, I64, I, I, F, P, P, P]),
+        "niq_tile_shape": (I, [I, I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]),
         "niq_eval_points": (I, [P, P, I64, I, P, P]),
         "niq_tree_workspace_bytes": (I, [P, P, P, ctypes.POINTER(S)]),
         "niq_tree_build": (I, [P, P, P, P, S, P, P]),
@@ -147,6 +148,7 @@ def load_library():
         "niq_sample_surface_nodes": (I, [P, P, P, I64, F, ctypes.c_uint64, I64, I64, I64, P, I64,
                                          ctypes.POINTER(I64), ctypes.POINTER(I64), P, S, P]),
         "niq_bulk_sums_nodes": (I, [P, P, P, I64, P, P, I64, I64, ctypes.c_uint64, P, P]),
+        "niq_camera_rays": (I, [P, P, P, P, I, F, P, P, P]),
         "niq_hit_positions": (I, [P, P, P, I64, P, P]),
         "niq_outward_normals": (I, [P, I, P, P, I64, F, I, P, P]),
         "niq_shade_image": (I, [P, P, P, P, I64, P, P]),

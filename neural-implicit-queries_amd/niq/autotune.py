@@ -1,32 +1,44 @@
-"""Kernel-variant autotuner for the classify pass.
+"""Tile autotuner for the classify pass and the ray caster.
 
-The fused classify kernel comes in variants chosen by ``EvalCfg.variant``:
+The search space is the kernels' tile shapes, each a compiled variant:
 
-* plain FP32 (interval and affine_fixed over 3 box vectors): 1 is the
-  default tile (``TileCfg``: 16-32 boxes per workgroup), 2 the alternative
-  tile (``TileCfgAlt``: twice the boxes per workgroup where LDS allows, fewer
-  waves at width 512);
-* precision mode (BF16): 1 streams the packed bf16 weight copies, 2 loads the
-  FP32 weights once and splits them in registers, 3 runs layer by layer
-  (one batched launch per hidden layer over a chunk of boxes, activations
-  through HBM).
+* plain FP32 classify (interval and affine_fixed over 3 box vectors),
+  ``EvalCfg.variant``: 1 the default tile (``TileCfg``: 16-32 boxes per
+  workgroup, 4-8 waves, the register budget LDS occupancy allows), 2 twice
+  the boxes per workgroup where LDS allows (fewer waves at width 512), 3 the
+  default tile with a 256-register budget (no spills, fewer resident
+  workgroups);
+* precision mode (BF16) classify: 1 streams the packed bf16 weight copies,
+  2 loads the FP32 weights once and splits them in registers, 3 runs layer
+  by layer (one batched launch per hidden layer, activations through HBM);
+* the ray caster, ``EvalCfg.ray_variant``: 0 the default tile, 1 the same
+  tile with a 256-register budget, 2 16-slot tiles at widths up to 128 and
+  32-slot tiles at width 256 (256 registers).
 
-Variants compute the same bounds: the plain FP32 tiles sum every (box,
-feature) in the same order (bit-identical), the precision-mode weight sources
-give the same splits (the e-plane bound differs by at most 2^-15 relative)
-and variant 3 repeats variant 2's arithmetic exactly.  ``tune`` times each
-variant on boxes of the size the finest tree cells of ``split_depth`` have,
-keeps the fastest in ``func.cfg.variant`` and caches the choice per (layer
-widths, activations, mode, precision).
+``niq_tile_shape`` reports each variant's (boxes or slots per workgroup,
+waves, waves per SIMD of the register budget).  Variants compute the same
+results: the plain FP32 tiles and the ray tiles sum every (box, feature) in
+the same order (bit-identical); the precision-mode weight sources give the
+same splits (the e-plane bound differs by at most 2^-15 relative) and
+variant 3 repeats variant 2's arithmetic exactly.  ``tune`` times the
+classify variants on boxes of the size of the finest tree cells of
+``split_depth``; ``tune_rays`` times the ray variants on a camera's rays.
+Both keep the fastest in the function's config and cache the choice per
+(layer widths, activations, mode, precision).
 """
+import ctypes
+
 import numpy as np
 import torch
 
+from . import _lib
 from . import mlp as _mlp
 
-VARIANTS = (1, 2)
+VARIANTS = (1, 2, 3)
 VARIANTS_BF16 = (1, 2, 3)
+RAY_VARIANTS = (0, 1, 2)
 _cache = {}
+_ray_cache = {}
 
 
 def variants_of(func):
@@ -39,10 +51,32 @@ def _key(func, params):
     return (tuple(m.dims), acts, int(func.cfg.mode), int(func.cfg.precision))
 
 
+def tile_shape(kernel, variant, width):
+    """{"boxes_per_workgroup", "waves", "waves_per_simd"} of a variant:
+    kernel "classify" (plain FP32) or "rays"."""
+    lib = _lib.load_library()
+    nb, nw, occ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _lib.check(lib.niq_tile_shape({"classify": 0, "rays": 1}[kernel], int(variant), int(width),
+                                  ctypes.byref(nb), ctypes.byref(nw), ctypes.byref(occ)))
+    return {"boxes_per_workgroup": nb.value, "waves": nw.value, "waves_per_simd": occ.value}
+
+
+def _time(fn, reps):
+    fn()                                  # warm-up (and first-launch set-up)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
 def tune(func, params, split_depth=18, n_boxes=1 << 17, reps=3, seed=0):
-    """Time every variant of ``func``'s classify pass on ``n_boxes`` random
-    cells of half-width 2^-(split_depth/3) in [-1, 1]^3; set and return the
-    fastest as {variant: ms}."""
+    """Time every classify variant of ``func`` on ``n_boxes`` random cells of
+    half-width 2^-(split_depth/3) in [-1, 1]^3; set the fastest and return
+    {"best", "ms": {variant: ms}, "shape"}."""
     key = _key(func, params)
     if key in _cache:
         func.cfg.variant = _cache[key]["best"]
@@ -56,18 +90,34 @@ def tune(func, params, split_depth=18, n_boxes=1 << 17, reps=3, seed=0):
     saved = func.cfg.variant
     for v in variants_of(func):
         func.cfg.variant = v
-        func.classify_box(params, lo, up)          # warm-up (and first-launch set-up)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            func.classify_box(params, lo, up)
-        e1.record()
-        torch.cuda.synchronize()
-        times[v] = e0.elapsed_time(e1) / reps
+        times[v] = _time(lambda: func.classify_box(params, lo, up), reps)
     func.cfg.variant = saved
     best = min(times, key=times.get)
     func.cfg.variant = best
     res = {"best": best, "ms": times, "n_boxes": n_boxes}
+    if int(func.cfg.precision) == 0:
+        res["shape"] = tile_shape("classify", best, max(_mlp.device_model(params).dims))
     _cache[key] = res
+    return res
+
+
+def tune_rays(func, params, roots, dirs, opts, reps=2):
+    """Time every ray-caster variant of ``func`` on the given rays; set the
+    fastest in ``func.cfg.ray_variant`` and return {"best", "ms", "shape"}."""
+    from . import queries
+    key = _key(func, params)
+    if key in _ray_cache:
+        func.cfg.ray_variant = _ray_cache[key]["best"]
+        return _ray_cache[key]
+    times = {}
+    saved = func.cfg.ray_variant
+    for v in RAY_VARIANTS:
+        func.cfg.ray_variant = v
+        times[v] = _time(lambda: queries.cast_rays((func,), (params,), roots, dirs, opts), reps)
+    func.cfg.ray_variant = saved
+    best = min(times, key=times.get)
+    func.cfg.ray_variant = best
+    res = {"best": best, "ms": times, "n_rays": int(roots.shape[0]),
+           "shape": tile_shape("rays", best, max(_mlp.device_model(params).dims))}
+    _ray_cache[key] = res
     return res

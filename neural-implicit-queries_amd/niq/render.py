@@ -1,9 +1,10 @@
 """Camera rays, normals, shading and images (src/render.py:17-253).
 
-Rays are formed in float32 on the host exactly as the reference forms them,
-then moved to the GPU.  Hit positions, finite-difference normals, shading,
-compositing and the tonemap run on the gfx950 kernels (niq_hit_positions,
-niq_outward_normals, niq_shade_image).
+Camera rays are formed on the GPU (niq_camera_rays) in the reference's
+float32 operations, or on the host when asked for host arrays.  Hit
+positions, finite-difference normals, shading, compositing and the tonemap
+run on the gfx950 kernels (niq_hit_positions, niq_outward_normals,
+niq_shade_image).
 """
 import ctypes
 
@@ -45,9 +46,34 @@ def look_at(eye_pos, target=None, up_dir='y'):
     return look_dir, up, left_dir
 
 
+def _camera_frame(look_dir, up_dir, fov_deg):
+    """The per-camera part of src/render.py:26-50 (float32): look, up made
+    orthogonal to look and normalised, left = look x up, tan(fov / 2)."""
+    look_dir = np.asarray(look_dir, f32)
+    up_dir = np.asarray(up_dir, f32)
+    up_dir = up_dir - np.dot(look_dir, up_dir) * look_dir
+    up_dir = _normalize(up_dir)
+    left_dir = np.cross(look_dir, up_dir).astype(f32)
+    tan_half = f32(np.tan(np.deg2rad(f32(fov_deg)) / f32(2)))
+    return look_dir, up_dir, left_dir, tan_half
+
+
 def generate_camera_rays(eye_pos, look_dir, up_dir, res=1024, fov_deg=30., device="cuda"):
     """src/render.py:26-50: (roots, dirs) of shape (res*res, 3); pixel
-    p = row * res + col over linspace(-1, 1, res)^2 ('xy' meshgrid)."""
+    p = row * res + col over linspace(-1, 1, res)^2 ('xy' meshgrid).  On a
+    device the rays are formed there (niq_camera_rays, the same float32
+    operations); with device=None as host arrays."""
+    if device is not None:
+        _lib.require_gpu()
+        lib = _lib.load_library()
+        look, up, left, tan_half = _camera_frame(look_dir, up_dir, fov_deg)
+        vec = lambda v: (ctypes.c_float * 3)(*[float(x) for x in np.asarray(v, f32).reshape(-1)])
+        roots = torch.empty((res * res, 3), dtype=torch.float32, device=device)
+        dirs = torch.empty((res * res, 3), dtype=torch.float32, device=device)
+        with torch.cuda.device(roots.device):
+            _lib.check(lib.niq_camera_rays(vec(eye_pos), vec(look), vec(up), vec(left), int(res), float(tan_half),
+                                           _lib.ptr(roots), _lib.ptr(dirs), _lib.stream_handle()))
+        return roots, dirs
     ax = np.linspace(-1., 1., res, dtype=f32)
     cam_x, cam_y = np.meshgrid(ax, ax)
     cam_x = cam_x.reshape(-1)
@@ -62,9 +88,7 @@ def generate_camera_rays(eye_pos, look_dir, up_dir, res=1024, fov_deg=30., devic
            + up_dir[None, :] * (cam_y * tan_half)[:, None])
     dirs = _normalize(pos.astype(f32))
     roots = np.tile(np.asarray(eye_pos, f32), (dirs.shape[0], 1))
-    if device is None:
-        return roots, dirs
-    return torch.as_tensor(roots, device=device), torch.as_tensor(dirs, device=device)
+    return roots, dirs
 
 
 def camera_ray(look_dir, up_dir, left_dir, fov_deg_x, fov_deg_y, theta_x, theta_y):

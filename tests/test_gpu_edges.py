@@ -139,3 +139,46 @@ def test_shallow_trees_match_oracle(depth):
         np.testing.assert_array_equal(v, ref[f"{key}_node_valid"])
         np.testing.assert_array_equal(out[f"{key}_node_lower"].cpu().numpy()[v], ref[f"{key}_node_lower"][v])
         np.testing.assert_array_equal(out[f"{key}_node_upper"].cpu().numpy()[v], ref[f"{key}_node_upper"][v])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_large_cast_writes_every_ray(precision):
+    """A cast of far more rays than the persistent grid holds slots (2^20 rays
+    of a 1024^2 camera; the grid holds a few tens of thousands): every output
+    is written (buffers prefilled with a sentinel) and equals the same rays
+    cast in 16 chunks, bit for bit, so no ray is left unclaimed or claimed
+    twice whatever the order in which slots free up."""
+    import ctypes
+    from niq import _lib, mlp as nmlp
+    note(f"[large cast] 2^20 rays, {precision}")
+    p = model("c1_relu_3x32")
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed", precision=precision)
+    roots, dirs = render.generate_camera_rays(np.array([0., 0., 3.], np.float32), *render.look_at((0., 0., 3.))[:2],
+                                              res=1024, fov_deg=60.0)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    lib = _lib.load_library()
+    m = nmlp.device_model(p)
+    handles = (ctypes.c_void_p * 1)(m.handle.value)
+    o = queries.cast_opts_struct(opts)
+    n = roots.shape[0]
+    t = torch.full((n,), -7.0, device="cuda")
+    h = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    c = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    ws_bytes = lib.niq_cast_workspace_bytes(ctypes.byref(o))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    ne = ctypes.c_int64(0)
+    torch.cuda.synchronize()
+    _lib.check(lib.niq_cast_rays(handles, 1, ctypes.byref(func.cfg), _lib.ptr(roots), _lib.ptr(dirs), n,
+                                 ctypes.byref(o), _lib.ptr(t), _lib.ptr(h), _lib.ptr(c), _lib.ptr(ws), ws_bytes,
+                                 ctypes.byref(ne), _lib.stream_handle()))
+    torch.cuda.synchronize()
+    unwritten = int(((t == -7.0) | (h == -7) | (c == -7)).sum().item())
+    note(f"  unwritten rays {unwritten}; hits {int((h > 0).sum().item())}; N_evals {ne.value}")
+    assert unwritten == 0
+    assert int((c < 1).sum().item()) == 0
+    k = n // 16
+    for i in range(16):
+        ct, ch, cc, _ = queries.cast_rays((func,), (p,), roots[i * k:(i + 1) * k], dirs[i * k:(i + 1) * k], opts)
+        assert torch.equal(ct, t[i * k:(i + 1) * k]) and torch.equal(ch, h[i * k:(i + 1) * k])
+        assert torch.equal(cc, c[i * k:(i + 1) * k])

@@ -1074,10 +1074,37 @@ def test_tile_variants_are_identical(name, mode):
         f.cfg.variant = v
         c, b = f.classify_box(p, lo, up, return_bounds=True)
         out[v] = (c.cpu().numpy(), b.cpu().numpy())
-    assert np.array_equal(out[1][0], out[2][0]) and np.array_equal(out[1][1], out[2][1])
+    for v in autotune.VARIANTS[1:]:
+        assert np.array_equal(out[1][0], out[v][0]) and np.array_equal(out[1][1], out[v][1]), v
     res = autotune.tune(f, p, split_depth=12, n_boxes=1 << 14, reps=2)
     note(f"  autotune {res}")
     assert f.cfg.variant == res["best"] and res["best"] in autotune.VARIANTS
+    assert res["shape"]["boxes_per_workgroup"] in (16, 32, 64) and res["shape"]["waves"] in (4, 8, 16)
+
+
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "c2_relu_256x6", "fox"])
+@pytest.mark.parametrize("mode", ["affine_fixed", "slope_interval"])
+def test_ray_variants_are_identical(name, mode):
+    """The ray-caster tiles differ only in slots per workgroup, waves and
+    register budget: every (ray, feature) sums in the same order, so the rays
+    are bit-identical; the autotuner keeps the fastest."""
+    from niq import autotune
+    note(f"[ray variants] {name} {mode}")
+    p = model(name)
+    f, _ = implicit_mlp_utils.implicit_from_params(p, mode)
+    roots, dirs = camera(48)
+    roots, dirs = torch.as_tensor(roots, device="cuda"), torch.as_tensor(dirs, device="cuda")
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    out = []
+    for v in autotune.RAY_VARIANTS:
+        f.cfg.ray_variant = v
+        out.append(queries.cast_rays((f,), (p,), roots, dirs, opts))
+    for v, o in zip(autotune.RAY_VARIANTS[1:], out[1:]):
+        assert all(torch.equal(a, b) for a, b in zip(out[0][:3], o[:3])) and out[0][3] == o[3], v
+    res = autotune.tune_rays(f, p, roots, dirs, opts, reps=1)
+    note(f"  autotune {res}")
+    assert f.cfg.ray_variant == res["best"]
 
 
 @pytest.mark.parametrize("name", ["c2_relu_256x6", "c4_relu_512x8", "fox"])
