@@ -192,6 +192,22 @@ niq_status niq_cast_rays(const niq_model* const* models, int n_models, const niq
                          int32_t* out_count, void* workspace, size_t workspace_bytes,
                          int64_t* n_evals, void* stream);
 
+/* ---- ray casting without allocation -------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc, processes
+ * with graph capture in flight): niq_cast_rays with its workspace held in a
+ * static device array of NIQ_CAST_NOALLOC_SCRATCH_BYTES on each GPU instead of
+ * a workspace argument.  Calls no hipMalloc or hipFree.  The limit:
+ * niq_cast_workspace_bytes(opts) <= NIQ_CAST_NOALLOC_SCRATCH_BYTES, that is
+ * n_max_step up to 10,000; larger requests fail with NIQ_ERR_CAPACITY.  Every call
+ * on one GPU shares that GPU's array, so calls that overlap on one GPU (from
+ * other threads or streams) race.  Synchronising, like niq_cast_rays. */
+#define NIQ_CAST_NOALLOC_SCRATCH_BYTES 65536
+niq_status niq_cast_rays_noalloc(const niq_model* const* models, int n_models, const niq_eval_cfg* cfg,
+                                 const float* roots, const float* dirs, int64_t n,
+                                 const niq_cast_opts* opts, float* out_t, int32_t* out_hit_id,
+                                 int32_t* out_count, int64_t* n_evals, void* stream);
+
 /* ---- frustum ray casting -----------------------------------------------------
  * Replaces cast_rays_frustum (src/queries.py:454-587) with its iteration
  * (:178-364), subdivision (:371-432) and output (:434-452).  The camera is the

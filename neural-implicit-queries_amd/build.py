@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "niq", "libniq.so")
 SOURCES = ["api_model.hip", "api_classify.hip", "api_classify_bf16.hip", "api_classify_rows.hip", "api_tree.hip", "api_rays.hip", "api_rays_rows.hip", "api_cp_mc.hip",
-           "api_serve.hip", "api_frustum.hip",
+           "api_serve.hip", "api_frustum.hip", "api_cast_noalloc.hip",
            "api_isect.hip", "api_sample.hip", "api_render.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
