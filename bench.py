@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=90,
                     help="requests in the serving trace (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=3.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-max-points", type=int, default=512,
+                    help="largest closest-point batch of the serving trace (the configuration's is 65,536)")
+    ap.add_argument("--no-cp-cost", action="store_true", help="skip the closest-point cost samples")
     return ap.parse_args()
 
 
@@ -200,16 +203,56 @@ def bf16_bound_quality(name, n_cells=512, seed=4):
             "determinate_bf16": int((c16 != 0).sum()), "determinate_fp64": int((c64 != 0).sum())}
 
 
-def serving_trace(n_req, rate, seed=0, max_side=512, max_points=65536):
+def closest_point_cost(eps=0.05, samples=(("c1", 4096), ("c2", 16), ("c3", 16))):
+    """What one closest-point query costs on each serving shape: a batch of
+    `n` uniform points in [-1,1]^3 (eps 0.05) on the first resident model of
+    the shape, timed once after a warm-up.  Every node popped from the stack
+    runs one box forward (5 rows) and 7 point forwards, so a node is
+    12 rows through the layers; the FLOP rate is against the FP32 MFMA peak.
+    ``s_per_65536_points`` extrapolates the per-point time linearly to the
+    largest batch of the serving configuration."""
+    import torch
+    from niq import models, implicit_mlp_utils, kd_tree
+    from niq import mlp as nmlp
+    resident, _ = models.serving_models()
+    rng = np.random.default_rng(0)
+    lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    out = {}
+    for prefix, n in samples:
+        name, p = [r for r in resident if r[0].startswith(prefix)][0]
+        f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+        q = torch.as_tensor(rng.uniform(-1, 1, (n, 3)).astype(np.float32), device="cuda")
+        kd_tree.closest_point(f, p, lo, up, q[:1], eps=eps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kd_tree.closest_point(f, p, lo, up, q, eps=eps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        nodes = kd_tree.closest_point.last_n_processed
+        dims = list(nmlp.device_model(p).dims)
+        fl = models.flops_per_box(dims, rows=12)
+        out[name] = {"points": n, "nodes": nodes, "ms": dt * 1e3, "nodes_per_point": nodes / n,
+                     "nodes_per_s": nodes / dt, "rounds_of_2048": -(-nodes // 2048),
+                     "us_per_round": dt * 1e6 / -(-nodes // 2048), "flops_per_node": fl,
+                     "tflops": nodes * fl / dt / 1e12, "frac_fp32_mfma": nodes * fl / dt / 1e12 / PEAK_FP32_TFLOPS,
+                     "s_per_65536_points": dt / n * 65536}
+        progress(f"closest point {name}: {n} points {dt * 1e3:.1f} ms, {nodes} nodes")
+    return out
+
+
+def serving_trace(n_req, rate, seed=0, max_side=512, max_points=512):
     """The serving configuration: eight resident MLPs (seeds 10-17, shapes
     drawn from the three single-GPU models) under affine_fixed; Poisson
     arrivals (seed 0) of 50 % ray-cast tiles (side U{64..max_side} of the
     60-degree camera at distance 3), 30 % closest-point batches
     (U{1..max_points} points in [-1,1]^3, eps 0.05) and 20 % level-set trees
     (split depth U{12..21}); memory cap 200 GB.  n_req requests (the
-    configuration asks for 10,000) at `rate` requests/s.  Reports p50/p99
-    latency per kind and overall, requests/s and how many ray requests were
-    merged into shared casts."""
+    configuration asks for 10,000) at `rate` requests/s.  The configuration's
+    closest-point batches reach 65,536 points; on the randomly initialised
+    C2/C3-shaped models one point pops about 10^5 nodes, so a batch of 65,536
+    is minutes of GPU work on its own (``closest_point_cost``) and max_points
+    defaults to 512.  Reports p50/p99 latency per kind and overall,
+    requests/s and how many ray requests were merged into shared casts."""
     import torch
     from niq import models, implicit_mlp_utils, queries, render, serve
     resident_models, rng = models.serving_models(seed)
@@ -543,8 +586,10 @@ def main():
     if not args.no_concurrent:
         line["concurrent_trees"] = concurrent_trees()
         progress(f"concurrent trees: {line['concurrent_trees']['boxes_per_s_served']:.4g} boxes/s served")
+    if not args.no_cp_cost:
+        line["closest_point"] = closest_point_cost()
     if args.serve_requests > 0:
-        line["serving"] = serving_trace(args.serve_requests, args.serve_rate)
+        line["serving"] = serving_trace(args.serve_requests, args.serve_rate, max_points=args.serve_max_points)
         progress(f"serving: {line['serving']['requests_per_s']:.4g} requests/s")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(params, args.cpu_seconds)

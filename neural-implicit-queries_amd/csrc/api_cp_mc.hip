@@ -9,6 +9,57 @@ template <int W>
 niq_status launch_points_dev_w(const niq_model* m, const float* x, const int* n_dev, int mult, long long max_n,
                                float* out, hipStream_t s);
 
+// A closest-point round as two launches: k_cp_forward (box and point tiles of
+// 16, grid sized for a full batch) and the one-workgroup update.
+template <int KIND, int V, int W>
+niq_status launch_cp_round_w(const niq_model* m, CpState* S, const CpArgs& a, hipStream_t s) {
+  if constexpr (!Fits<Planes<KIND, V, 0>::R, 16, W>::value) {
+    return fail(NIQ_ERR_UNSUPPORTED, "closest_point round: box planes above LDS");
+  } else {
+    constexpr int NW = TileCfgWide<W>::NW;
+    const long long g_box = (a.B + 15) / 16, g_pts = (7LL * a.B + 15) / 16;
+    hipLaunchKernelGGL((k_cp_forward<KIND, V, NW, W>), dim3((unsigned)(g_box + g_pts)), dim3(NW * 64), 0, s,
+                       m->dev, S, a, (int)g_box);
+    hipLaunchKernelGGL(k_cp_update, dim3(1), dim3(1024), 0, s, S, a);
+    HIPCHK(hipGetLastError());
+    return NIQ_OK;
+  }
+}
+
+template <int KIND, int V>
+niq_status launch_cp_round_kv(const niq_model* m, CpState* S, const CpArgs& a, hipStream_t s) {
+  switch (m->wt) {
+    case 64: return launch_cp_round_w<KIND, V, 64>(m, S, a, s);
+    case 128: return launch_cp_round_w<KIND, V, 128>(m, S, a, s);
+    case 256: return launch_cp_round_w<KIND, V, 256>(m, S, a, s);
+    case 512: return launch_cp_round_w<KIND, V, 512>(m, S, a, s);
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
+
+// The (KIND, V) form of the fused forward for this model and config, or -1:
+// it covers the FP32 interval, affine_fixed and slope_interval classifiers at
+// their default tile (variant 0) whose box planes fit LDS; other configs run
+// the round as gather, classify, points and update launches.
+int cp_round_form(const niq_model* m, const niq_eval_cfg* cfg) {
+  const int mode = cfg ? cfg->mode : NIQ_MODE_AFFINE_FIXED;
+  if (cfg && (cfg->precision != NIQ_PREC_FP32 || cfg->variant != 0)) return -1;
+  const size_t plane = 16 * (size_t)m->wt * 4;
+  if (mode == NIQ_MODE_INTERVAL) return 0;
+  if (mode == NIQ_MODE_AFFINE_FIXED) return (5 * plane <= (size_t)kLdsLimit) ? 1 : -1;
+  if (mode == NIQ_MODE_SLOPE_INTERVAL) return (7 * plane <= (size_t)kLdsLimit) ? 2 : -1;
+  return -1;
+}
+
+niq_status launch_cp_round(int form, const niq_model* m, CpState* S, const CpArgs& a, hipStream_t s) {
+  switch (form) {
+    case 0: return launch_cp_round_kv<0, 0>(m, S, a, s);
+    case 1: return launch_cp_round_kv<0, 3>(m, S, a, s);
+    case 2: return launch_cp_round_kv<1, 3>(m, S, a, s);
+  }
+  return fail(NIQ_ERR_INVALID, "closest_point round form");
+}
+
 }  // namespace
 
 extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* cfg, const float lower[3],
@@ -22,8 +73,8 @@ extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* 
   if (B < 1 || B > (1 << 20)) return fail(NIQ_ERR_INVALID, "batch_process_size out of range");
   if (Q == 0) return NIQ_OK;
   hipStream_t s = (hipStream_t)stream;
-  const int K = 8;                                  // rounds between host checks
-  long long cap = std::max<long long>(2 * Q, Q + 2LL * K * B);
+  const int K = cp_rounds_per_check(B);
+  long long cap = cp_initial_stack(Q, B);
   // per-call device scratch (stream ordered); every allocation made is freed
   // on every path, the stack arrays (which may be replaced as they grow) at the
   // end through `a`
@@ -104,8 +155,13 @@ extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* 
   ClassifyArgs ca{};
   ca.lower = a.blo; ca.upper = a.bup; ca.d = 3; ca.n_dev = &S->n; ca.offset = 0.f; ca.codes = a.codes;
   const unsigned gg = (unsigned)std::min<long long>((B + 255) / 256, 1024);
+  const int form = cp_round_form(m, cfg);
   while (true) {
     for (int r = 0; r < K && st == NIQ_OK; ++r) {
+      if (form >= 0) {
+        st = launch_cp_round(form, m, S, a, s);
+        continue;
+      }
       hipLaunchKernelGGL(k_cp_gather, dim3(gg), dim3(256), 0, s, S, a);
       if ((st = classify_dispatch(m, cfg, 3, ca, B, s))) break;
       switch (m->wt) {

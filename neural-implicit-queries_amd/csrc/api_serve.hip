@@ -502,8 +502,8 @@ niq_status niq_serve_submit_closest_point(niq_server* srv, const niq_model* m, c
   r->q = query_points; r->Q = Q; r->eps = eps; r->B = B;
   r->out_dist = out_dist; r->out_loc = out_loc;
   // per-round buffers, the query's winner ids and a node stack with room to
-  // grow twice (the stack starts at max(2Q, Q + 16B) entries of 28 bytes)
-  const size_t stack = (size_t)std::max<int64_t>(2 * Q, Q + 16LL * B);
+  // grow twice (28 bytes an entry)
+  const size_t stack = (size_t)cp_initial_stack(Q, B);
   r->need = 152 * (size_t)B + 4 * (size_t)Q + 28 * 4 * stack + 4096;
   return srv->submit(std::move(r), request_id);
 }

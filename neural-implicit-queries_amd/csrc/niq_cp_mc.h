@@ -13,6 +13,8 @@
 // cases of its 8^s subcells and writes their triangles in the reference's
 // (leaf, subcell, slot) order at offsets from a device prefix sum.
 #pragma once
+#include <rocprim/block/block_scan.hpp>
+
 #include "niq_kernels.h"
 
 // ------------------------------------------------------------ closest point
@@ -75,15 +77,23 @@ static __global__ void __launch_bounds__(256) k_cp_gather(CpState* S, CpArgs a) 
 
 __device__ __forceinline__ float norm3(float x, float y, float z) { return sqrtf((x * x + y * y) + z * z); }
 
-// One workgroup: distances, scatter-min, location, compaction and push
-// (src/kd_tree.py:692-760).
-static __global__ void __launch_bounds__(1024) k_cp_update(CpState* S, CpArgs a) {
-  const int n = S->n;
-  const int pop = S->pop;
-  if (n == 0) return;
+// A value another thread of this round changed with an atomic, read past the
+// caches (device scope).
+template <class T>
+__device__ __forceinline__ T load_fresh(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The round's update over the n popped items, by the NT threads of one
+// workgroup (src/kd_tree.py:692-760): distances, scatter-min, location, and
+// the stable compaction with the interleaved push at the pop position.
+template <int NT>
+__device__ __forceinline__ void cp_update_body(CpState* S, const CpArgs& a, int n, int pop) {
+  using Scan = rocprim::block_scan<int, NT>;
+  __shared__ typename Scan::storage_type scan_storage;
   // phase 1: per-item distance and subdivision flag (pruning uses the minimum
   // from before this round's scatter)
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int j = threadIdx.x; j < n; j += NT) {
     const int qid = a.bq[j];
     float lo[3], up[3], c[3], e[3];
     for (int k = 0; k < 3; ++k) {
@@ -105,47 +115,38 @@ static __global__ void __launch_bounds__(1024) k_cp_update(CpState* S, CpArgs a)
     }
     const bool spans = !(all_neg || all_pos);
     a.idist[j] = spans ? maxd : INFINITY;
-    a.ineeds[j] = (!outside && !is_small && dc < a.qmin[qid]) ? 1 : 0;
+    a.ineeds[j] = (!outside && !is_small && dc < load_fresh(&a.qmin[qid])) ? 1 : 0;
   }
   __syncthreads();
   // phase 2: scatter-min (non-negative floats order as their bit patterns)
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int j = threadIdx.x; j < n; j += NT) {
     const float d = a.idist[j];
     if (d < INFINITY) atomicMin(reinterpret_cast<int*>(&a.qmin[a.bq[j]]), __float_as_int(d));
   }
   __syncthreads();
   // phase 3: items equal to the new minimum claim the location; the last
   // such batch entry wins
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int j = threadIdx.x; j < n; j += NT) {
     const int qid = a.bq[j];
-    if (a.idist[j] == a.qmin[qid]) atomicMax(&a.win[qid], j);
+    if (a.idist[j] == load_fresh(&a.qmin[qid])) atomicMax(&a.win[qid], j);
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int j = threadIdx.x; j < n; j += NT) {
     const int qid = a.bq[j];
-    if (a.win[qid] == j) {
+    if (load_fresh(&a.win[qid]) == j) {
       for (int k = 0; k < 3; ++k) a.qloc[3 * qid + k] = 0.5f * (a.blo[3 * j + k] + a.bup[3 * j + k]);
     }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < n; j += blockDim.x) a.win[a.bq[j]] = -1;
+  for (int j = threadIdx.x; j < n; j += NT) a.win[a.bq[j]] = -1;
   // phase 4: stable compaction of the subdivided items, children pushed
   // interleaved A0 B0 A1 B1 ... at the pop position
-  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int per = (n + NT - 1) / NT;
   const int j0 = threadIdx.x * per, j1 = min(j0 + per, n);
   int cnt = 0;
   for (int j = j0; j < j1; ++j) cnt += a.ineeds[j];
-  __shared__ int sc[1024];
-  sc[threadIdx.x] = cnt;
-  __syncthreads();
-  for (int w = 1; w < (int)blockDim.x; w <<= 1) {
-    const int v = (int)threadIdx.x >= w ? sc[threadIdx.x - w] : 0;
-    __syncthreads();
-    sc[threadIdx.x] += v;
-    __syncthreads();
-  }
-  int r = sc[threadIdx.x] - cnt;
-  const int total = sc[blockDim.x - 1];
+  int r = 0, total = 0;
+  Scan().exclusive_scan(cnt, r, 0, total, scan_storage);
   for (int j = j0; j < j1; ++j) {
     if (!a.ineeds[j]) continue;
     float lo[3], up[3];
@@ -172,6 +173,90 @@ static __global__ void __launch_bounds__(1024) k_cp_update(CpState* S, CpArgs a)
     long long nt = (long long)pop + 2LL * total;
     if (nt > a.cap) { S->overflow = 1; nt = a.cap; }
     S->top = (int)nt;
+  }
+}
+
+// One workgroup applies the update of the round k_cp_gather popped.
+static __global__ void __launch_bounds__(1024) k_cp_update(CpState* S, CpArgs a) {
+  const int n = S->n;
+  if (n == 0) return;
+  cp_update_body<1024>(S, a, n, S->pop);
+}
+
+// The sample point t of a node: its centre plus (upper - lower) times the
+// offset row t of [0; I; -I] (src/kd_tree.py:703-704).
+__device__ __forceinline__ float cp_sample(const float* lo, const float* up, int t, int k) {
+  const float c = 0.5f * (lo[k] + up[k]);
+  const float e = up[k] - lo[k];
+  const float o = (t == 0) ? 0.f : ((t - 1) % 3 == k ? (t <= 3 ? 1.f : -1.f) : 0.f);
+  return c + e * o;
+}
+
+// The forward half of a closest-point round in one launch (the update is
+// k_cp_update, the next launch on the stream).  Workgroups [0, g_box) take 16
+// popped nodes each through the box forward (Planes<KIND, V>, the forward of
+// k_classify) and copy them to the batch buffers; the others take 16 of the
+// 7n sample points each through the point forward (the forward of
+// k_points_dev), so box and point tiles fill the chip together.  Every
+// (node, feature) and (point, feature) sums in the order of those kernels:
+// codes and values are theirs bit for bit.
+template <int KIND, int V, int NW, int WMAX>
+__global__ void __launch_bounds__(NW * 64, (LdsOcc<Planes<KIND, V, 0>::R, 16, NW, WMAX>::value))
+k_cp_forward(const DevModel* __restrict__ Mp, CpState* S, CpArgs a, int g_box) {
+  using PB = Planes<KIND, V, 0>;
+  using PP = Planes<2, 0, 1>;
+  constexpr int NB = 16, NT = NW * 64;
+  __shared__ float X[PB::R * NB * WMAX];
+  const int top = S->top;
+  const int pop = max(top - a.B, 0);
+  const int n = top - pop;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->n = n;
+    S->pop = pop;
+    S->n_processed += n;
+    if (n > 0) S->rounds += 1;
+  }
+  if ((int)blockIdx.x < g_box) {
+    const long long i0 = (long long)blockIdx.x * NB;
+    if (i0 >= n) return;
+    for (int b = threadIdx.x; b < NB; b += NT) {
+      const long long i = i0 + b;
+      if (i < n) {
+        const long long s = pop + i;
+        a.bq[i] = a.wq[s];
+        for (int k = 0; k < 3; ++k) { a.blo[3 * i + k] = a.wlo[3 * s + k]; a.bup[3 * i + k] = a.wup[3 * s + k]; }
+      }
+    }
+    ClassifyArgs ca{};
+    ca.lower = a.wlo + 3LL * pop;
+    ca.upper = a.wup + 3LL * pop;
+    ca.d = 3;
+    fill_box_inputs<KIND, V, NB, NT, WMAX>(X, ca, i0, n);
+    mlp_forward<PB, KIND, V, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+      const long long i = i0 + b;
+      if (i >= n) return;
+      float lo, hi, scale;
+      final_bounds<KIND, V>(v, lo, hi, scale);
+      a.codes[i] = sign_code(lo, hi, 0.f);
+    });
+  } else {
+    const long long p0 = (long long)(blockIdx.x - g_box) * NB;
+    const long long np = 7LL * n;
+    if (p0 >= np) return;
+    for (int e = threadIdx.x; e < NB * 16; e += NT) {
+      const int b = e >> 4, k = e & 15;
+      const long long q = p0 + b;
+      float val = 0.f;
+      if (q < np && k < 3) {
+        const long long s = pop + q / 7;
+        val = cp_sample(a.wlo + 3 * s, a.wup + 3 * s, (int)(q % 7), k);
+      }
+      X[xoff<NB, WMAX>(0, b, k)] = val;
+    }
+    mlp_forward<PP, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+      const long long q = p0 + b;
+      if (q < np) a.vals[q] = v[0];
+    });
   }
 }
 

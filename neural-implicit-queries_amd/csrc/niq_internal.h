@@ -67,6 +67,13 @@ inline int template_width(int wmax) {
   return -1;
 }
 
+// closest_point: rounds between host checks of the stack (up to 32 while the
+// room they need, 2 K B rows, stays small) and the stack's initial rows.
+inline int cp_rounds_per_check(int B) { return (int)std::clamp<long long>((1LL << 16) / B, 2, 32); }
+inline long long cp_initial_stack(long long Q, int B) {
+  return std::max<long long>(2 * Q, Q + 2LL * cp_rounds_per_check(B) * B);
+}
+
 inline int num_cus() {
   static int cus = -1;
   if (cus < 0) {

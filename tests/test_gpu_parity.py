@@ -926,6 +926,31 @@ def test_closest_point_matches_oracle(name, nq, eps, batch):
     np.testing.assert_array_equal(loc[fin], rloc[fin])
 
 
+@pytest.mark.parametrize("name,mode,nq,eps,batch", [
+    ("c1_relu_3x32", "affine_fixed", 300, 0.01, 2048), ("c1_relu_3x32", "interval", 64, 0.05, 256),
+    ("c1_relu_3x32", "slope_interval", 64, 0.05, 2048), ("small_tanh", "affine_fixed", 64, 0.05, 100),
+    ("c2_relu_256x6", "affine_fixed", 8, 0.1, 2048), ("c2_relu_256x6", "slope_interval", 4, 0.1, 2048),
+    ("c4_relu_512x8", "affine_fixed", 1, 0.25, 2048), ("c4_relu_512x8", "interval", 1, 0.25, 512)])
+def test_closest_point_fused_round_equals_split(name, mode, nq, eps, batch):
+    """The default config runs each round's forward as one launch of box and
+    point tiles (then the update); a classify tile variant runs it as gather,
+    classify, points and update launches.  Same sums in the same order:
+    distances, locations and the node count are identical."""
+    note(f"[closest point fused] {name} {mode} batch {batch}")
+    p = model(name)
+    q = torch.as_tensor(np.random.default_rng(5).uniform(-1, 1, size=(nq, 3)).astype(np.float32), device="cuda")
+    out = {}
+    for variant in (0, 1):
+        func, _ = implicit_mlp_utils.implicit_from_params(p, mode)
+        func.cfg.variant = variant
+        d, loc = kd_tree.closest_point(func, p, LO, UP, q, eps=eps, batch_process_size=batch)
+        out[variant] = (d.cpu().numpy(), loc.cpu().numpy(), kd_tree.closest_point.last_n_processed)
+    note(f"  {out[0][2]} nodes")
+    assert out[0][2] == out[1][2]
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 def test_closest_point_plane_closed_form():
     nrm = np.array([0., 0., 1.], np.float32)
     A1 = np.concatenate([np.eye(3), -np.eye(3)], 1)
