@@ -380,10 +380,11 @@ def c3_affine_all(depth=21):
     """TanH MLP 3->128x8->1 (Xavier init, seed 1) under affine_all, whose
     error terms grow to 3 + 1024 rows: the level-set tree to `depth`."""
     import torch
-    from niq import models, implicit_mlp_utils, kd_tree
+    from niq import models, implicit_mlp_utils, kd_tree, autotune
     p = models.config_model("c3_tanh_128x8")
     f, _ = implicit_mlp_utils.implicit_from_params(p, "affine_all")
     lo, up = np.full(3, -1, np.float32), np.full(3, 1, np.float32)
+    tuned = autotune.tune(f, p, split_depth=depth, n_boxes=1 << 14, reps=1)
     kd_tree.build_tree(f, p, lo, up, split_depth=9)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -396,19 +397,42 @@ def c3_affine_all(depth=21):
     eb = models.error_bytes_per_box_affine_all(dims)
     tflops = fl * st["n_classified"] / dt / 1e12
     gbs = eb * st["n_classified"] / dt / 1e9
+    # dominant launch: the classify of the largest level's box count, timed
+    # alone on torch's stream with HIP events (affine_all's work per box does
+    # not depend on the box: every box carries 3 + 128 rows per activation)
+    n_big = int(max(lv))
+    g = torch.Generator(device="cpu").manual_seed(0)
+    c = torch.rand((n_big, 3), generator=g) * 2 - 1
+    h = torch.full((n_big, 3), 2.0 ** -7)
+    blo, bup = (c - h).cuda(), (c + h).cuda()
+    f.classify_box(p, blo, bup)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(3):
+        f.classify_box(p, blo, bup)
+    ev1.record()
+    torch.cuda.synchronize()
+    launch_ms = ev0.elapsed_time(ev1) / 3
+    l_tflops = fl * n_big / (launch_ms * 1e-3) / 1e12
+    l_gbs = eb * n_big / (launch_ms * 1e-3) / 1e9
     return {"workload": f"levelset tree, TanH MLP 3->128x8->1, affine_all, split depth {depth}",
             "seconds": dt, "boxes": st["n_classified"], "boxes_per_s": st["n_classified"] / dt,
             "unknown_leaves": st["n_unknown"], "active_boxes_per_level": [min(lv), max(lv)],
+            "variant": tuned["best"], "tile": tuned.get("shape"), "variant_ms": tuned["ms"],
             # the whole tree's algorithmic work over its wall time: the
             # error-term contraction (FP32 MFMA) and the error-term matrix
             # streamed through HBM (read and written once per layer)
             "roofline": {"kernel": "k_classify_rows (affine_all, W=128)", "flops_per_box": fl,
                          "achieved_tflops": tflops, "peak_tflops": PEAK_FP32_TFLOPS,
                          "frac_mfma": tflops / PEAK_FP32_TFLOPS, "hbm_bytes_per_box": eb,
-                         "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS, "frac_hbm": gbs / PEAK_HBM_GBS}}
+                         "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS, "frac_hbm": gbs / PEAK_HBM_GBS,
+                         "largest_level_launch": {"boxes": n_big, "launch_ms": launch_ms,
+                                                  "achieved_tflops": l_tflops,
+                                                  "frac_mfma": l_tflops / PEAK_FP32_TFLOPS,
+                                                  "achieved_gbs": l_gbs, "frac_hbm": l_gbs / PEAK_HBM_GBS}}}
 
 
-def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):
+def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1, n_streams=4):
     """64 level-set trees over 16 resident TanH MLPs (Xavier init, seeds
     20-35; widths from {32, 64, 128, 256} and 3-8 hidden layers drawn from
     seed 1), affine_fixed with autotuned kernel variants, built one after
@@ -434,8 +458,8 @@ def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):
         flops += st["n_classified"] * models.flops_per_box([3] + [w] * nl + [1], rows=5)
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
-    with serve.Server(n_streams=4) as srv:
-        # first pass captures each worker's graphs; the timed pass replays them
+    with serve.Server(n_streams=n_streams) as srv:
+        # the first pass captures the graphs; the timed pass replays them
         warm = [srv.submit_tree(fams[i % n_models][0], fams[i % n_models][1], lo, up, split_depth=depth)
                 for i in range(n_trees)]
         for r in warm:
@@ -458,6 +482,11 @@ def concurrent_trees(n_models=16, n_trees=64, depth=15, seed=1):
             "tflops_sequential": flops / t_seq / 1e12, "tflops_served": flops / t_srv / 1e12,
             "frac_fp32_mfma_served": flops / t_srv / 1e12 / PEAK_FP32_TFLOPS,
             "latency_ms": lat.get("tree"),
+            # requests per shared build (trees of one config, depth and
+            # template width waiting together run their levels in shared
+            # launches)
+            "batch_sizes": sorted({int(r.info["n_batched"]) for r in reqs}),
+            "trees_from_graphs": sum(1 for r in reqs if r.info["graph_replay"]),
             "variants": [int(f.cfg.variant) for f, _, _, _ in fams],
             "tiles": [r.get("shape") for r in tuned],
             "shapes": [f"{w}x{nl}" for _, _, w, nl in fams]}

@@ -67,9 +67,13 @@ typedef struct niq_eval_cfg {
   float recheck_tol;   /* > 0: FP64 re-test of cells within tol*scale of the threshold */
   int variant;         /* kernel variant, 0 = default; plain FP32: 1 TileCfg,
                           2 TileCfgAlt, 3 TileCfg with 256 registers
-                          (niq_tile_shape); precision mode: 1 streams the
+                          (niq_tile_shape); growing-row modes: 2 half the
+                          error-term rows per chunk, 3 a 2-waves-per-SIMD
+                          register budget; precision mode: 1 streams the
                           packed bf16 weight copies, 2 splits FP32 weights in
-                          registers (the default), 3 layer by layer */
+                          registers (the default), 3 layer by layer; closest
+                          point runs its round as one forward launch only at
+                          variant 0.  Every variant gives the same results. */
   float sdf_lipschitz; /* NIQ_MODE_SDF: Lipschitz bound of f (sdf_lipschitz kwarg) */
   int ray_variant;     /* cast_rays tile shape, 0 = default; 1 and 2 are the
                           alternatives the autotuner times (same results) */
@@ -108,9 +112,11 @@ niq_status niq_classify_general_boxes(const niq_model* m, const niq_eval_cfg* cf
 
 /* Tile shape of a kernel variant, for autotuners and reports: which 0 is the
  * plain FP32 classify pass (eval cfg variant 0-3), 1 the ray caster
- * (ray_variant 0-2); width a layer width of the model (its template width is
- * used).  *nb: boxes or ray slots per workgroup, *nw: waves per workgroup,
- * *occ: waves per SIMD the register budget is set for.  Host only. */
+ * (ray_variant 0-2), 2 the growing-row classify (variant 0-3); width a layer
+ * width of the model (its template width is used).  *nb: boxes or ray slots
+ * per workgroup (which 2: error-term rows per chunk; one box per workgroup),
+ * *nw: waves per workgroup, *occ: waves per SIMD the register budget is set
+ * for.  Host only. */
 niq_status niq_tile_shape(int which, int variant, int width, int* nb, int* nw, int* occ);
 
 /* ---- plain evaluation ----------------------------------------------------
@@ -379,9 +385,14 @@ niq_status niq_shade_image(const niq_shade_opts* opts, const float* ray_dirs, co
  * requests and max_merge_rays rays): each request is a group of rays with its
  * own models, options, inputs and outputs inside one persistent kernel, and
  * every ray is independent, so merged results (N_evals included) equal lone
- * ones bit for bit.  Tree builds in the fused-kernel modes with a
- * split_depth replay a hipGraph captured once per (model, config, depth,
- * capacities) on each worker.  Closest-point batches run as they are.
+ * ones bit for bit.  Tree builds in the fused-kernel FP32 modes with a
+ * split_depth that wait in the queue with the same eval config, build params
+ * (root boxes aside) and template width are merged too (up to 64): one build
+ * runs every level of all of them in shared launches, each tree's level as
+ * its lone build runs it, so the outputs are a lone build's bit for bit.  A
+ * tree build taken alone replays a hipGraph captured once per (model,
+ * config, depth, capacities) and shared by the workers (a graph replays on
+ * one stream at a time).  Closest-point batches run as they are.
  *
  * Buffers: inputs must be complete on the device when submitted and stay
  * alive, and outputs untouched, until the request has been polled done.
@@ -399,7 +410,8 @@ typedef struct niq_serve_info {
   double queue_ms;       /* submit -> start */
   double latency_ms;     /* submit -> done */
   int64_t n_evals;       /* rays: the request's N_evals, as a lone cast gives it */
-  int32_t n_batched;     /* rays: requests merged into the cast that served it */
+  int32_t n_batched;     /* rays, trees: requests merged into the launches that
+                            served it (1: alone) */
   int32_t graph_replay;  /* trees: 0 built without a graph; 1 served by a graph
                             captured for this request; 2 by replaying a graph
                             cached from an earlier request */

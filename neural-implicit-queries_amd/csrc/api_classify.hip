@@ -221,6 +221,8 @@ niq_status tile_shape_w(int which, int variant, int* nb, int* nw, int* occ) {
       case 2: shape_of<TileCfgAlt<W>, W>(5, nb, nw, occ); return NIQ_OK;
       case 3: shape_of<TileCfgRegs<W>, W>(5, nb, nw, occ); return NIQ_OK;
     }
+  } else if (which == 2) {   // growing-row classify: *nb is the error-term rows per chunk
+    return rows_tile_shape(variant, W, nb, nw, occ);
   } else if (which == 1) {   // ray caster, affine_fixed (5 planes)
     switch (variant) {
       case 0: shape_of<RayTile<W, 0>, W>(5, nb, nw, occ); return NIQ_OK;

@@ -7,11 +7,11 @@ namespace {
 
 // affine_all / affine_truncate / affine_append: one box per workgroup, error
 // terms streamed through a private HBM scratch.
-template <int W, int MODE>
+template <int W, int MODE, int OCC = RowsOcc<W>::value, int CH = RowsCfg<W>::CH>
 niq_status launch_rows_w(const niq_model* m, const niq_eval_cfg* cfg, const ClassifyArgs& a, long long max_n,
                          hipStream_t s) {
   constexpr int NW = RowsCfg<W>::NW;
-  auto k = k_classify_rows<W, MODE>;
+  auto k = k_classify_rows<W, MODE, OCC, CH>;
   const long long d0 = std::max(a.d, a.lower ? 0 : a.vin);
   const int n_keep = cfg ? cfg->truncate_count : 0;
   const int n_append = cfg ? cfg->n_append : 0;
@@ -41,9 +41,30 @@ niq_status launch_rows_w(const niq_model* m, const niq_eval_cfg* cfg, const Clas
   return NIQ_OK;
 }
 
+// Variants (niq_eval_cfg.variant): 0 and 1 the default (RowsCfg chunk, the
+// RowsOcc register budget); 2 half the rows per chunk (fewer registers, more
+// barriers); 3 at widths up to 128 a two-waves-per-SIMD budget (no spills,
+// half the resident workgroups; the wider templates already run at one or
+// two).  Every variant sums in the same order: identical results.
 template <int MODE>
 niq_status launch_rows(const niq_model* m, const niq_eval_cfg* cfg, const ClassifyArgs& a, long long max_n,
                        hipStream_t s) {
+  const int variant = cfg ? cfg->variant : 0;
+  if (variant == 2) {
+    switch (m->wt) {
+      case 64: return launch_rows_w<64, MODE, RowsOcc<64>::value, RowsCfg<64>::CH / 2>(m, cfg, a, max_n, s);
+      case 128: return launch_rows_w<128, MODE, RowsOcc<128>::value, RowsCfg<128>::CH / 2>(m, cfg, a, max_n, s);
+      case 256: return launch_rows_w<256, MODE, RowsOcc<256>::value, RowsCfg<256>::CH / 2>(m, cfg, a, max_n, s);
+      case 512: return launch_rows_w<512, MODE, RowsOcc<512>::value, RowsCfg<512>::CH / 2>(m, cfg, a, max_n, s);
+    }
+    return fail(NIQ_ERR_UNSUPPORTED, "width");
+  }
+  if (variant == 3) {
+    switch (m->wt) {
+      case 64: return launch_rows_w<64, MODE, 2>(m, cfg, a, max_n, s);
+      case 128: return launch_rows_w<128, MODE, 2>(m, cfg, a, max_n, s);
+    }
+  }
   switch (m->wt) {
     case 64: return launch_rows_w<64, MODE>(m, cfg, a, max_n, s);
     case 128: return launch_rows_w<128, MODE>(m, cfg, a, max_n, s);
@@ -68,3 +89,23 @@ niq_status niqi::classify_rows(const niq_model* m, const niq_eval_cfg* cfg, cons
     return fail(NIQ_ERR_INVALID, "affine_append needs 0 <= n_append <= 512");
   return launch_rows<ROWS_APPEND>(m, cfg, a, max_n, s);
 }
+
+namespace niqi {
+template <int W>
+static void rows_shape_w(int variant, int* ch, int* nw, int* occ) {
+  *ch = variant == 2 ? RowsCfg<W>::CH / 2 : RowsCfg<W>::CH;
+  *nw = RowsCfg<W>::NW;
+  *occ = (variant == 3 && W <= 128) ? 2 : RowsOcc<W>::value;
+}
+
+niq_status rows_tile_shape(int variant, int wt, int* ch, int* nw, int* occ) {
+  if (variant < 0 || variant > 3) return fail(NIQ_ERR_INVALID, "unknown variant");
+  switch (wt) {
+    case 64: rows_shape_w<64>(variant, ch, nw, occ); return NIQ_OK;
+    case 128: rows_shape_w<128>(variant, ch, nw, occ); return NIQ_OK;
+    case 256: rows_shape_w<256>(variant, ch, nw, occ); return NIQ_OK;
+    case 512: rows_shape_w<512>(variant, ch, nw, occ); return NIQ_OK;
+  }
+  return fail(NIQ_ERR_UNSUPPORTED, "width");
+}
+}  // namespace niqi

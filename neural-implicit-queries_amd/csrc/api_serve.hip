@@ -77,6 +77,18 @@ bool same_cast(const Request& a, const Request& b) {
          req_wt(a, false) == req_wt(b, false) && req_wt(a, true) == req_wt(b, true);
 }
 
+// Tree requests that can share one batched build: the same eval config and
+// build params (root boxes aside), one template width, a batchable form.
+bool same_tree_batch(const Request& a, const Request& b) {
+  if (a.kind != REQ_TREE || b.kind != REQ_TREE) return false;
+  if (std::memcmp(&a.cfg, &b.cfg, sizeof(niq_eval_cfg)) != 0 || a.model->wt != b.model->wt) return false;
+  const niq_tree_params &p = a.tp, &q = b.tp;
+  return p.split_depth == q.split_depth && p.node_terminate_thresh == q.node_terminate_thresh &&
+         p.batch_process_size == q.batch_process_size && p.with_interior_nodes == q.with_interior_nodes &&
+         p.with_exterior_nodes == q.with_exterior_nodes && p.offset == q.offset &&
+         tree_batchable(b.model, &b.cfg, &b.tp);
+}
+
 // One captured tree build: its workspace, internal outputs and root box stay
 // allocated while the graph is cached.
 struct TreeGraph {
@@ -95,6 +107,7 @@ struct TreeGraph {
   float* fin_e = nullptr;
   float* root = nullptr;   // 6 floats
   hipGraphExec_t exec = nullptr;
+  bool busy = false;       // checked out by a worker replaying it
 
   bool matches(const Request& r) const {
     return model_serial == r.model->serial && std::memcmp(&cfg, &r.cfg, sizeof(cfg)) == 0 && split_depth == r.tp.split_depth &&
@@ -111,7 +124,8 @@ struct TreeGraph {
   }
 };
 
-constexpr int kGraphsPerWorker = 4;
+// Tree graphs the server keeps (besides the byte cap of 1/8 of its memory cap).
+constexpr int kMaxGraphs = 64;
 
 }  // namespace
 
@@ -126,6 +140,10 @@ struct niq_server {
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   size_t in_use = 0;        // reserved by running requests
   size_t graph_bytes = 0;   // held by cached graphs
+  // Captured tree builds shared by all workers, most recently used first.  A
+  // worker checks one out (busy) for a replay, so one instance never runs on
+  // two streams at once; a key in use everywhere gets another instance.
+  std::list<TreeGraph> graphs;
   int running = 0;
   bool stop = false;
   uint64_t next_id = 1;
@@ -141,7 +159,6 @@ struct Worker {
   niq_server* srv = nullptr;
   hipStream_t s = nullptr;
   void* pinned = nullptr;   // root box (256 B) + tree counters
-  std::list<TreeGraph> graphs;   // most recently used first
 
   void finish(Request* r, niq_status st) {
     r->status = st;
@@ -150,9 +167,10 @@ struct Worker {
 
   void run_rays(std::vector<Request*>& group);
   void run_tree(Request* r);
+  void run_tree_batch(std::vector<Request*>& group);
   void run_cp(Request* r);
   TreeGraph* graph_for(Request* r, bool* reused);
-  void clear_graphs();
+  void return_graph(TreeGraph* g);
 };
 
 void Worker::run_rays(std::vector<Request*>& group) {
@@ -189,11 +207,15 @@ void Worker::run_rays(std::vector<Request*>& group) {
 
 TreeGraph* Worker::graph_for(Request* r, bool* reused) {
   *reused = false;
-  for (auto it = graphs.begin(); it != graphs.end(); ++it) {
-    if (it->matches(*r)) {
-      graphs.splice(graphs.begin(), graphs, it);
-      *reused = true;
-      return &graphs.front();
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    for (auto it = srv->graphs.begin(); it != srv->graphs.end(); ++it) {
+      if (!it->busy && it->matches(*r)) {
+        srv->graphs.splice(srv->graphs.begin(), srv->graphs, it);
+        srv->graphs.front().busy = true;
+        *reused = true;
+        return &srv->graphs.front();
+      }
     }
   }
   size_t ws_bytes = 0;
@@ -206,16 +228,26 @@ TreeGraph* Worker::graph_for(Request* r, bool* reused) {
   g.cap_ext = g.with_ext ? r->tres.exterior_capacity : 0;
   g.ws_bytes = ws_bytes;
   g.bytes = ws_bytes + 24 * (size_t)(g.cap_unk + g.cap_int + g.cap_ext) + 256;
+  std::list<TreeGraph> evicted;   // released after the lock is dropped
   {
     std::lock_guard<std::mutex> lk(srv->mu);
-    while (srv->graph_bytes + g.bytes > srv->graph_cap && !graphs.empty()) {
-      srv->graph_bytes -= graphs.back().bytes;
-      graphs.back().release();
-      graphs.pop_back();
+    // least recently used idle graphs make room (count and bytes); a busy
+    // graph is held by a worker and stays
+    for (auto it = srv->graphs.end(); it != srv->graphs.begin();) {
+      if (srv->graph_bytes + g.bytes <= srv->graph_cap && (int)srv->graphs.size() < kMaxGraphs) break;
+      --it;
+      if (it->busy) continue;
+      srv->graph_bytes -= it->bytes;
+      auto victim = it++;   // it: the graph after the victim (or end)
+      evicted.splice(evicted.end(), srv->graphs, victim);
     }
-    if (srv->graph_bytes + g.bytes > srv->graph_cap) return nullptr;
+    if (srv->graph_bytes + g.bytes > srv->graph_cap) {
+      for (TreeGraph& e : evicted) e.release();
+      return nullptr;
+    }
     srv->graph_bytes += g.bytes;
   }
+  for (TreeGraph& e : evicted) e.release();
   bool ok = hipMalloc(&g.ws, ws_bytes) == hipSuccess &&
             hipMalloc((void**)&g.unk, 24 * (size_t)std::max<int64_t>(g.cap_unk, 1)) == hipSuccess &&
             hipMalloc((void**)&g.root, 256) == hipSuccess;
@@ -250,17 +282,19 @@ TreeGraph* Worker::graph_for(Request* r, bool* reused) {
     srv->graph_bytes -= g.bytes;
     return nullptr;
   }
-  graphs.push_front(g);
-  if ((int)graphs.size() > kGraphsPerWorker) {
-    std::lock_guard<std::mutex> lk(srv->mu);
-    srv->graph_bytes -= graphs.back().bytes;
-    graphs.back().release();
-    graphs.pop_back();
-  }
-  return &graphs.front();
+  g.busy = true;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  srv->graphs.push_front(g);
+  return &srv->graphs.front();
+}
+
+void Worker::return_graph(TreeGraph* g) {
+  std::lock_guard<std::mutex> lk(srv->mu);
+  g->busy = false;
 }
 
 void Worker::run_tree(Request* r) {
+  r->n_batched = 1;
   const int mode = r->cfg.mode;
   const bool capturable = r->tp.split_depth >= 0 && r->tp.node_terminate_thresh <= 0 &&
                           mode != NIQ_MODE_AFFINE_ALL && mode != NIQ_MODE_AFFINE_TRUNCATE &&
@@ -298,6 +332,7 @@ void Worker::run_tree(Request* r) {
       hipfail(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
     r->graph_replay = reused ? 2 : 1;
+    return_graph(g);
   } else {
     size_t ws_bytes = 0;
     void* ws = nullptr;
@@ -310,20 +345,39 @@ void Worker::run_tree(Request* r) {
   finish(r, st);
 }
 
+// Several tree builds merged: every level of all of them in shared launches.
+void Worker::run_tree_batch(std::vector<Request*>& group) {
+  const int ng = (int)group.size();
+  std::vector<const niq_model*> models(ng);
+  std::vector<float> roots(6 * ng);
+  std::vector<niq_tree_result*> res(ng);
+  for (int g = 0; g < ng; ++g) {
+    Request* r = group[g];
+    models[g] = r->model;
+    for (int k = 0; k < 3; ++k) { roots[6 * g + k] = r->tp.lower[k]; roots[6 * g + 3 + k] = r->tp.upper[k]; }
+    res[g] = &r->tres;
+  }
+  const Request& r0 = *group[0];
+  niq_status st = NIQ_OK;
+  size_t bytes = 0;
+  void* ws = nullptr;
+  st = tree_batch_bytes(r0.model, &r0.cfg, &r0.tp, ng, &bytes);
+  if (st == NIQ_OK && hipMallocAsync(&ws, bytes, s) != hipSuccess) st = fail(NIQ_ERR_HIP, "tree batch workspace");
+  if (st == NIQ_OK) st = tree_batch_build(models.data(), ng, &r0.cfg, &r0.tp, roots.data(), ws, bytes, res.data(), s);
+  if (ws) (void)hipFreeAsync(ws, s);
+  if (hipStreamSynchronize(s) != hipSuccess && st == NIQ_OK) st = fail(NIQ_ERR_HIP, "hipStreamSynchronize failed");
+  for (Request* r : group) {
+    r->n_batched = ng;
+    finish(r, st);
+  }
+}
+
 void Worker::run_cp(Request* r) {
+  r->n_batched = 1;
   niq_status st = niq_closest_point(r->model, &r->cfg, r->lower, r->upper, r->q, r->Q, r->eps, r->B, r->out_dist,
                                     r->out_loc, &r->n_processed, s);
   if (hipStreamSynchronize(s) != hipSuccess && st == NIQ_OK) st = fail(NIQ_ERR_HIP, "hipStreamSynchronize failed");
   finish(r, st);
-}
-
-void Worker::clear_graphs() {
-  std::lock_guard<std::mutex> lk(srv->mu);
-  for (TreeGraph& g : graphs) {
-    srv->graph_bytes -= g.bytes;
-    g.release();
-  }
-  graphs.clear();
 }
 
 }  // namespace
@@ -362,6 +416,24 @@ void niq_server::worker_main() {
             ++it;
           }
         }
+      } else if (r->kind == REQ_TREE && tree_batchable(r->model, &r->cfg, &r->tp)) {
+        // builds of the same config and depth run their levels together; the
+        // waiting ones are shared out over the workers, so that several
+        // batches overlap on the device (a batch's small levels are latency)
+        int compatible = 1;
+        for (const Request* o : queue) compatible += same_tree_batch(*r, *o) ? 1 : 0;
+        const int n_workers = std::max<int>(1, (int)workers.size());
+        const int target = std::min(kMaxTreeBatch, (compatible + n_workers - 1) / n_workers);
+        for (auto it = queue.begin(); it != queue.end() && (int)group.size() < target;) {
+          Request* o = *it;
+          if (same_tree_batch(*r, *o) && in_use + need + o->need <= cap) {
+            group.push_back(o);
+            need += o->need;
+            it = queue.erase(it);
+          } else {
+            ++it;
+          }
+        }
       }
       in_use += need;
       ++running;
@@ -373,7 +445,8 @@ void niq_server::worker_main() {
     } else if (group[0]->kind == REQ_RAYS) {
       w.run_rays(group);
     } else if (group[0]->kind == REQ_TREE) {
-      w.run_tree(group[0]);
+      if (group.size() > 1) w.run_tree_batch(group);
+      else w.run_tree(group[0]);
     } else {
       w.run_cp(group[0]);
     }
@@ -388,7 +461,6 @@ void niq_server::worker_main() {
     cv_done.notify_all();
     cv_work.notify_all();
   }
-  if (ready) w.clear_graphs();
   if (w.pinned) (void)hipHostFree(w.pinned);
   if (w.s) (void)hipStreamDestroy(w.s);
 }
@@ -435,6 +507,13 @@ niq_status niq_serve_destroy(niq_server* srv) {
   }
   srv->cv_work.notify_all();
   for (auto& t : srv->workers) t.join();
+  // no worker runs now: release the shared graphs on the server's device
+  int prev = 0;
+  const bool have_prev = hipGetDevice(&prev) == hipSuccess;
+  if (!srv->graphs.empty() && hipSetDevice(srv->device) == hipSuccess) {
+    for (TreeGraph& g : srv->graphs) g.release();
+    if (have_prev) (void)hipSetDevice(prev);
+  }
   delete srv;
   return NIQ_OK;
 }
@@ -479,7 +558,7 @@ niq_status niq_serve_submit_tree(niq_server* srv, const niq_model* m, const niq_
   r->tres = *res;
   size_t ws = 0;
   if (niq_status st = niq_tree_workspace_bytes(m, &r->cfg, p, &ws)) return st;
-  r->need = ws;
+  r->need = ws + 8192;   // and its share of a merged build's table of trees
   return srv->submit(std::move(r), request_id);
 }
 

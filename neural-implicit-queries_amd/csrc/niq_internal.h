@@ -138,6 +138,21 @@ inline niq_status check_model(const niq_model* m, int d) {
 
 // Classification of up to max_n boxes (count may be device-resident), routed
 // by (mode, precision, box vectors).  Defined in api_classify.hip.
+// Rows per chunk, waves and register budget (waves per SIMD) of a
+// growing-row classify variant at template width wt.
+niq_status rows_tile_shape(int variant, int wt, int* ch, int* nw, int* occ);
+// Batched level-set trees (the serving loop's merged builds): up to
+// kMaxTreeBatch trees with one eval config, one set of params (their root
+// boxes aside) and one template width, built level by level in shared
+// launches.  Each tree's outputs and counts equal its lone build's.
+constexpr int kMaxTreeBatch = 64;
+bool tree_batchable(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p);
+niq_status tree_batch_bytes(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, int ng,
+                            size_t* bytes);
+// roots: 6 floats per tree (lower then upper); synchronises.
+niq_status tree_batch_build(const niq_model* const* models, int ng, const niq_eval_cfg* cfg,
+                            const niq_tree_params* p, const float* roots, void* ws, size_t ws_bytes,
+                            niq_tree_result* const* res, hipStream_t s);
 niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec, const ClassifyArgs& a,
                              long long max_n, hipStream_t s);
 // Whether a classification of `nvec` box vectors keeps its planes in global

@@ -365,12 +365,13 @@ struct TreeCounters {
   int level_ext[64];
 };
 
-// Per batch of B nodes: counts of UNKNOWN / POSITIVE / NEGATIVE codes.
-static __global__ void __launch_bounds__(256)
-k_tree_count(const int* __restrict__ codes, const TreeCounters* __restrict__ C, int level, int B,
-             int* __restrict__ bcnt) {
+// Per batch of B nodes: counts of UNKNOWN / POSITIVE / NEGATIVE codes.  The
+// bodies of the tree-step kernels take their batch index (bx) as an argument,
+// so a batched build of several trees runs them with one tree per grid row.
+__device__ __forceinline__ void tree_count_body(const int* __restrict__ codes, const TreeCounters* __restrict__ C,
+                                                int level, int B, int* __restrict__ bcnt, int bx) {
   const int n = C->n[level & 1];
-  const long long start = (long long)blockIdx.x * B;
+  const long long start = (long long)bx * B;
   if (start >= n) return;
   const long long end = min(start + (long long)B, (long long)n);
   int cu = 0, cp = 0, cn = 0;
@@ -390,17 +391,23 @@ k_tree_count(const int* __restrict__ codes, const TreeCounters* __restrict__ C, 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    bcnt[blockIdx.x * 4 + 0] = s[0][0];
-    bcnt[blockIdx.x * 4 + 1] = s[1][0];
-    bcnt[blockIdx.x * 4 + 2] = s[2][0];
+    bcnt[bx * 4 + 0] = s[0][0];
+    bcnt[bx * 4 + 1] = s[1][0];
+    bcnt[bx * 4 + 2] = s[2][0];
   }
 }
 
-// Single workgroup: exclusive scan of the per-batch counts, level totals.
-// boff[b] = {unknown prefix, exterior offset, interior offset}.
-static __global__ void __launch_bounds__(1024)
-k_tree_scan(const int* __restrict__ bcnt, int* __restrict__ boff, TreeCounters* C, int level, int B,
-            int split, long long cap_next, long long cap_int, long long cap_ext) {
+static __global__ void __launch_bounds__(256)
+k_tree_count(const int* __restrict__ codes, const TreeCounters* __restrict__ C, int level, int B,
+             int* __restrict__ bcnt) {
+  tree_count_body(codes, C, level, B, bcnt, blockIdx.x);
+}
+
+// Single workgroup of 1024: exclusive scan of the per-batch counts, level
+// totals.  boff[b] = {unknown prefix, exterior offset, interior offset}.
+__device__ __forceinline__ void tree_scan_body(const int* __restrict__ bcnt, int* __restrict__ boff,
+                                               TreeCounters* C, int level, int B, int split, long long cap_next,
+                                               long long cap_int, long long cap_ext) {
   const int n = C->n[level & 1];
   const int nb = (int)((n + (long long)B - 1) / B);
   __shared__ int s[3][1024];
@@ -449,6 +456,12 @@ k_tree_scan(const int* __restrict__ bcnt, int* __restrict__ boff, TreeCounters* 
   }
 }
 
+static __global__ void __launch_bounds__(1024)
+k_tree_scan(const int* __restrict__ bcnt, int* __restrict__ boff, TreeCounters* C, int level, int B,
+            int split, long long cap_next, long long cap_int, long long cap_ext) {
+  tree_scan_body(bcnt, boff, C, level, B, split, cap_next, cap_int, cap_ext);
+}
+
 struct TreeScatterArgs {
   const int* codes;
   const float* lower; const float* upper;   // current level (n, 3)
@@ -469,17 +482,17 @@ __device__ __forceinline__ void store3(float* p, long long i, float x, float y, 
 // reference's order (src/kd_tree.py:46-96, src/bucketing.py:16-32).  The batch
 // is walked in chunks of 256 consecutive nodes, one per thread (coalesced
 // reads and writes); ranks within a chunk come from wave ballots.
-static __global__ void __launch_bounds__(256)
-k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __restrict__ boff,
-               const TreeCounters* __restrict__ C, int level, int B, int split) {
+__device__ __forceinline__ void tree_scatter_body(const TreeScatterArgs& a, const int* __restrict__ bcnt,
+                                                  const int* __restrict__ boff, const TreeCounters* __restrict__ C,
+                                                  int level, int B, int split, int bx) {
   const int n = C->n[level & 1];
-  const long long start = (long long)blockIdx.x * B;
+  const long long start = (long long)bx * B;
   if (start >= n) return;
   const int cnt = (int)min((long long)B, (long long)n - start);
-  const long long Ub = boff[blockIdx.x * 4 + 0];
-  const long long nUb = bcnt[blockIdx.x * 4 + 0];
-  const long long Eb = boff[blockIdx.x * 4 + 1];
-  const long long Ib = boff[blockIdx.x * 4 + 2];
+  const long long Ub = boff[bx * 4 + 0];
+  const long long nUb = bcnt[bx * 4 + 0];
+  const long long Eb = boff[bx * 4 + 1];
+  const long long Ib = boff[bx * 4 + 2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
   __shared__ int wsum[3][4];
@@ -540,6 +553,12 @@ k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __res
     }
     ru0 += tu; rp0 += tp; rn0 += tn;
   }
+}
+
+static __global__ void __launch_bounds__(256)
+k_tree_scatter(TreeScatterArgs a, const int* __restrict__ bcnt, const int* __restrict__ boff,
+               const TreeCounters* __restrict__ C, int level, int B, int split) {
+  tree_scatter_body(a, bcnt, boff, C, level, B, split, blockIdx.x);
 }
 
 // --------------------------------------------------------- ray casting

@@ -69,10 +69,15 @@ __device__ __forceinline__ int block_argmax(const float* vals, const unsigned ch
 // per CU at width <= 128).
 template <int WMAX> struct RowsOcc { static constexpr int value = WMAX <= 128 ? 4 : 1; };
 
-template <int WMAX, int MODE>
-__global__ void __launch_bounds__(RowsCfg<WMAX>::NW * 64, RowsOcc<WMAX>::value)
+// OCC: the register budget in waves per SIMD (RowsOcc by default; 2 trades
+// resident workgroups for registers).  CH: rows per chunk of the error-term
+// stream (a multiple of 16; each row's products and each lane's column sums
+// run in row order whatever the chunk, so every CH gives the same bits).
+template <int WMAX, int MODE, int OCC = RowsOcc<WMAX>::value, int CH = RowsCfg<WMAX>::CH>
+__global__ void __launch_bounds__(RowsCfg<WMAX>::NW * 64, OCC)
 k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
-  constexpr int CH = RowsCfg<WMAX>::CH, NW = RowsCfg<WMAX>::NW;
+  static_assert(CH % 16 == 0, "chunks of whole 16-row tiles");
+  constexpr int NW = RowsCfg<WMAX>::NW;
   constexpr int NT = NW * 64;
   constexpr int BT = CH / 16;
   constexpr int FTMAX = (WMAX / 16 + NW - 1) / NW;

@@ -11,14 +11,19 @@ The search space is the kernels' tile shapes, each a compiled variant:
 * precision mode (BF16) classify: 1 streams the packed bf16 weight copies,
   2 loads the FP32 weights once and splits them in registers, 3 runs layer
   by layer (one batched launch per hidden layer, activations through HBM);
+* the growing-row classify (affine_all, affine_truncate, affine_append; one
+  box per workgroup, its error terms streamed in chunks): 1 the default, 2
+  half the error-term rows per chunk, 3 a two-waves-per-SIMD register budget
+  at widths up to 128;
 * the ray caster, ``EvalCfg.ray_variant``: 0 the default tile, 1 the same
   tile with a 256-register budget, 2 16-slot tiles at widths up to 128 and
   32-slot tiles at width 256 (256 registers).
 
-``niq_tile_shape`` reports each variant's (boxes or slots per workgroup,
-waves, waves per SIMD of the register budget).  Variants compute the same
-results: the plain FP32 tiles and the ray tiles sum every (box, feature) in
-the same order (bit-identical); the precision-mode weight sources give the
+``niq_tile_shape`` reports each variant's (boxes or slots per workgroup, or
+error-term rows per chunk; waves; waves per SIMD of the register budget).
+Variants compute the same results: the plain FP32 tiles, the growing-row
+chunks and the ray tiles sum every (box, feature) in the same order
+(bit-identical); the precision-mode weight sources give the
 same splits (the e-plane bound differs by at most 2^-15 relative) and
 variant 3 repeats variant 2's arithmetic exactly.  ``tune`` times the
 classify variants on boxes of the size of the finest tree cells of
@@ -51,14 +56,20 @@ def _key(func, params):
     return (tuple(m.dims), acts, int(func.cfg.mode), int(func.cfg.precision))
 
 
+ROWS_MODES = (_lib.MODE["affine_truncate"], _lib.MODE["affine_append"], _lib.MODE["affine_all"])
+
+
 def tile_shape(kernel, variant, width):
     """{"boxes_per_workgroup", "waves", "waves_per_simd"} of a variant:
-    kernel "classify" (plain FP32) or "rays"."""
+    kernel "classify" (plain FP32), "rays" or "rows" (growing-row classify:
+    {"error_rows_per_chunk", "waves", "waves_per_simd"}, one box per
+    workgroup)."""
     lib = _lib.load_library()
     nb, nw, occ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    _lib.check(lib.niq_tile_shape({"classify": 0, "rays": 1}[kernel], int(variant), int(width),
+    _lib.check(lib.niq_tile_shape({"classify": 0, "rays": 1, "rows": 2}[kernel], int(variant), int(width),
                                   ctypes.byref(nb), ctypes.byref(nw), ctypes.byref(occ)))
-    return {"boxes_per_workgroup": nb.value, "waves": nw.value, "waves_per_simd": occ.value}
+    first = "error_rows_per_chunk" if kernel == "rows" else "boxes_per_workgroup"
+    return {first: nb.value, "waves": nw.value, "waves_per_simd": occ.value}
 
 
 def _time(fn, reps):
@@ -96,7 +107,8 @@ def tune(func, params, split_depth=18, n_boxes=1 << 17, reps=3, seed=0):
     func.cfg.variant = best
     res = {"best": best, "ms": times, "n_boxes": n_boxes}
     if int(func.cfg.precision) == 0:
-        res["shape"] = tile_shape("classify", best, max(_mlp.device_model(params).dims))
+        kernel = "rows" if int(func.cfg.mode) in ROWS_MODES else "classify"
+        res["shape"] = tile_shape(kernel, best, max(_mlp.device_model(params).dims))
     _cache[key] = res
     return res
 

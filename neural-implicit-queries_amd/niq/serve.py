@@ -9,15 +9,19 @@ inside the library, each with its own HIP stream:
   under ``mem_cap_bytes`` beside the work in flight (arrival order is kept);
 * continuous batching: ray requests waiting with the same models, config and
   cast options are merged into the cast of the request taken;
-* tree builds with a ``split_depth`` in the fused-kernel modes replay a hipGraph
-  captured once per (model, config, depth, capacities) on each worker;
+* tree builds with a ``split_depth`` in the fused-kernel modes that wait in the
+  queue together (same config, depth and template width, any models) run as
+  one batched build, every level of all of them in shared launches; a lone
+  one replays a hipGraph captured once per (model, config, depth,
+  capacities), shared by the workers;
 * every request reports its queueing time and latency.
 
 Each result equals the result of the matching standalone call
 (``queries.cast_rays``, ``kd_tree.construct_uniform_unknown_levelset_tree``,
 ``kd_tree.closest_point``): the same kernels run on the same inputs, and rays
-are independent of one another.  Only ``N_evals`` of a merged cast counts the
-whole cast (``info['n_batched']`` requests).
+are independent of one another, and a batched tree build runs each tree's
+levels exactly as its lone build does.  ``info['n_batched']`` is the number
+of requests that shared the launches.
 """
 import ctypes
 

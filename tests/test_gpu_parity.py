@@ -1082,6 +1082,33 @@ def test_truncate_append_match_oracle(mode, kw, name):
 
 
 # ------------------------------------------------------------ kernel variants
+@pytest.mark.parametrize("name", ["c1_relu_3x32", "c3_tanh_128x8", "small_sin"])
+@pytest.mark.parametrize("mode,kw", [("affine_all", {}),
+                                     ("affine_truncate", {"affine_n_truncate": 8, "affine_truncate_policy": "absolute"}),
+                                     ("affine_append", {"affine_n_append": 4})])
+def test_rows_variants_are_identical(name, mode, kw):
+    """The growing-row kernel's variants differ in register budget and grid
+    size only: every box is one workgroup summing in the same order, so codes
+    and bounds are bit-identical."""
+    note(f"[rows variants] {name} {mode}")
+    p = model(name)
+    f, _ = implicit_mlp_utils.implicit_from_params(p, mode, **kw)
+    lo, up = boxes(300, seed=7, scale=0.3)
+    lo, up = torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda")
+    out = {}
+    for v in (0, 2, 3):
+        f.cfg.variant = v
+        c, b = f.classify_box(p, lo, up, return_bounds=True)
+        out[v] = (c.cpu().numpy(), b.cpu().numpy())
+    for v in (2, 3):
+        assert np.array_equal(out[0][0], out[v][0]) and np.array_equal(out[0][1], out[v][1]), v
+    if mode == "affine_all":
+        from niq import autotune
+        res = autotune.tune(f, p, split_depth=12, n_boxes=1 << 10, reps=1)
+        note(f"  autotune {res}")
+        assert f.cfg.variant == res["best"] and res["shape"]["error_rows_per_chunk"] in (16, 32, 64)
+
+
 @pytest.mark.parametrize("name", ["c1_relu_3x32", "c2_relu_256x6", "c4_relu_512x8", "fox"])
 @pytest.mark.parametrize("mode", ["affine_fixed", "interval"])
 def test_tile_variants_are_identical(name, mode):

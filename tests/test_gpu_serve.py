@@ -3,9 +3,10 @@
 Bar: identical outputs, bit for bit.  A served request runs the same kernels
 on the same inputs as the standalone call: rays are independent of one
 another (a merged cast gives every ray what a lone cast gives it, each
-request being its own group of rays with its own models and options), and
-each tree build and closest-point batch runs as one call of its own.  Checked with
-requests merged (one worker kept busy so that ray requests queue), with three
+request being its own group of rays with its own models and options), a
+batched tree build runs each tree's levels as its lone build does, and each
+closest-point batch runs as one call of its own.  Checked with requests
+merged (one worker kept busy so that ray and tree requests queue), with three
 workers overlapping on the device, with tree builds replayed from a captured
 hipGraph over different root boxes, with memory admission serialising
 requests that do not fit together, and with requests that cannot be replayed
@@ -20,6 +21,7 @@ import torch
 
 from niq import implicit_mlp_utils, kd_tree, queries, render, models, serve
 from niq import _lib
+from niq import mlp as nmlp
 
 pytestmark = pytest.mark.gpu
 
@@ -152,41 +154,71 @@ def test_mix_equals_standalone():
         cp = srv.submit_closest_point(f2, p2, LO, UP, q, eps=0.05)
         for i, r in t_rays:
             assert same_rays(alone_rays[i], r.result()), f"ray request {i}"
-        replays = 0
+        replays, batched = 0, 0
         for i, r in t_trees:
             assert same_tree(alone_trees[i], r.result()), f"tree request {i}"
-            replays += r.info["graph_replay"]
+            # a tree waits in the queue with others of its config (merged
+            # into one batched build) or runs alone (a captured graph)
+            assert r.info["graph_replay"] > 0 or r.info["n_batched"] > 1, i
+            replays += r.info["graph_replay"] > 0
+            batched += r.info["n_batched"] > 1
         d, loc = cp.result()
         assert torch.equal(d, alone_cp[0]) and torch.equal(loc, alone_cp[1])
         lat = serve.latency_summary([r for _, r in t_rays] + [r for _, r in t_trees] + [cp])
-        assert srv.bytes_in_use() > 0          # cached graphs stay resident
-    note(f"  graph replays {replays} of {len(t_trees)} trees; latency {lat}")
-    assert replays == len(t_trees)
+        if replays:
+            assert srv.bytes_in_use() > 0      # cached graphs stay resident
+    note(f"  {replays} of {len(t_trees)} trees from graphs, {batched} in batched builds; latency {lat}")
+
+
+def test_tree_requests_merge_into_one_build():
+    """Tree requests waiting behind a running one, with different models and
+    root boxes but one config, depth and template width, run as one batched
+    build; each result equals its lone build bit for bit."""
+    note("[serve] batched tree builds")
+    (f1, p1), (f2, p2), _ = setup()
+    assert max(nmlp.device_model(p1).dims) <= 64 and max(nmlp.device_model(p2).dims) <= 64   # one template width
+    boxes = [(LO, UP), (np.array([-1, -1, -1], np.float32), np.array([0, 0, 0], np.float32)),
+             (np.array([-0.5, -0.25, -1], np.float32), np.array([0.5, 0.75, 0], np.float32))]
+    kw = dict(split_depth=11, with_interior_nodes=True, with_exterior_nodes=True)
+    plan = [(f, p, lo, up) for (f, p) in ((f1, p1), (f2, p2)) for lo, up in boxes]
+    alone = [kd_tree.construct_uniform_unknown_levelset_tree(f, p, lo, up, **kw) for f, p, lo, up in plan]
+    with serve.Server(n_streams=1) as srv:
+        busy = srv.submit_tree(f1, p1, LO, UP, split_depth=16)
+        reqs = [srv.submit_tree(f, p, lo, up, **kw) for f, p, lo, up in plan]
+        busy.result()
+        got = [r.result() for r in reqs]
+        sizes = [r.info["n_batched"] for r in reqs]
+    note(f"  batch sizes {sizes}")
+    assert max(sizes) >= 3
+    for i, (a, g) in enumerate(zip(alone, got)):
+        assert same_tree(a, g), f"tree request {i}"
 
 
 def test_memory_admission_serialises():
     note("[serve] memory admission")
     (f1, p1), _, _ = setup()
-    alone = kd_tree.construct_uniform_unknown_levelset_tree(f1, p1, LO, UP, split_depth=16)
+    alone = kd_tree.construct_uniform_unknown_levelset_tree(f1, p1, LO, UP, split_depth=18)
     lib = _lib.load_library()
-    m, p, cap, cap_fin = kd_tree.tree_request(f1, p1, LO, UP, split_depth=16)
+    m, p, cap, cap_fin = kd_tree.tree_request(f1, p1, LO, UP, split_depth=18)
     import ctypes
     need = ctypes.c_size_t()
     _lib.check(lib.niq_tree_workspace_bytes(m.handle, ctypes.byref(f1.cfg), ctypes.byref(p), ctypes.byref(need)))
-    # room for one build at a time (graphs take 1/8 of the cap: too little here)
+    # room for one build at a time (graphs take 1/8 of the cap: too little
+    # here, and two builds do not fit one merged build either)
     with serve.Server(n_streams=2, mem_cap_bytes=int(need.value * 1.5)) as srv:
         t0 = time.perf_counter()
-        a = srv.submit_tree(f1, p1, LO, UP, split_depth=16)
-        t1 = time.perf_counter()
-        b = srv.submit_tree(f1, p1, LO, UP, split_depth=16)
+        a = srv.submit_tree(f1, p1, LO, UP, split_depth=18)
+        b = srv.submit_tree(f1, p1, LO, UP, split_depth=18)
+        gap_ms = (time.perf_counter() - t0) * 1e3   # at least b's submit time after a's
         ra, rb = a.result(), b.result()
         assert not a.info["graph_replay"] and not b.info["graph_replay"]
-        done_a = (t0 - t0) * 1e3 + a.info["latency_ms"]
-        start_b = (t1 - t0) * 1e3 + b.info["queue_ms"]
-        note(f"  a done at {done_a:.3f} ms, b started at {start_b:.3f} ms")
-        assert start_b >= done_a - 0.5
+        assert a.info["n_batched"] == 1 and b.info["n_batched"] == 1
+        run_a = a.info["latency_ms"] - a.info["queue_ms"]
+        note(f"  a ran {run_a:.3f} ms; b waited {b.info['queue_ms']:.3f} ms, submitted within {gap_ms:.3f} ms of a")
+        # b starts only once a is done: it waits a's run less the submit gap
+        assert b.info["queue_ms"] >= run_a - gap_ms - 0.2
         with pytest.raises(_lib.CapacityError):
-            srv.submit_tree(f1, p1, LO, UP, split_depth=18)
+            srv.submit_tree(f1, p1, LO, UP, split_depth=20)
     assert same_tree(alone, ra) and same_tree(alone, rb)
 
 
