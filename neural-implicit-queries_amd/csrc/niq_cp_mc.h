@@ -84,38 +84,132 @@ __device__ __forceinline__ T load_fresh(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Push the two children of a subdivided item at rank r of the stable
+// compaction: split along the longest axis (first on ties), interleaved
+// A0 B0 A1 B1 ... at the pop position (src/kd_tree.py:735-757).
+__device__ __forceinline__ void cp_push_children(const CpArgs& a, int pop, int r, int qid, const float* lo,
+                                                 const float* up) {
+  const float e0 = up[0] - lo[0], e1 = up[1] - lo[1], e2 = up[2] - lo[2];
+  int dim = 0; float em = e0;
+  if (e1 > em) { dim = 1; em = e1; }
+  if (e2 > em) dim = 2;
+  const float mid = 0.5f * (lo[dim] + up[dim]);
+  const long long pa = (long long)pop + 2LL * r, pb = pa + 1;
+  if (pb < a.cap) {
+    for (int k = 0; k < 3; ++k) {
+      a.wlo[3 * pa + k] = lo[k];
+      a.wup[3 * pa + k] = (k == dim) ? mid : up[k];
+      a.wlo[3 * pb + k] = (k == dim) ? mid : lo[k];
+      a.wup[3 * pb + k] = up[k];
+    }
+    a.wq[pa] = qid;
+    a.wq[pb] = qid;
+  }
+}
+
+// The per-item part of the update: the distance the item offers its query
+// (its centre distance plus the node diagonal when the samples change sign)
+// and whether it is subdivided (src/kd_tree.py:692-716).
+__device__ __forceinline__ void cp_item(const CpArgs& a, int j, int qid, const float* lo, const float* up,
+                                        float& idist, int& needs) {
+  float c[3], e[3];
+  for (int k = 0; k < 3; ++k) { c[k] = 0.5f * (lo[k] + up[k]); e[k] = up[k] - lo[k]; }
+  const float width = fmaxf(fmaxf(e[0], e[1]), e[2]);
+  const float diag = norm3(e[0], e[1], e[2]);
+  const float dc = norm3(a.q[3 * qid] - c[0], a.q[3 * qid + 1] - c[1], a.q[3 * qid + 2] - c[2]);
+  const bool is_small = width < a.eps_cube_width;
+  const int code = a.codes[j];
+  const bool outside = code == CODE_NEGATIVE || code == CODE_POSITIVE;
+  bool all_neg = true, all_pos = true;
+  for (int t = 0; t < 7; ++t) {
+    const float v = a.vals[7 * j + t];
+    all_neg = all_neg && (v < 0.f);
+    all_pos = all_pos && (v > 0.f);
+  }
+  idist = !(all_neg || all_pos) ? dc + diag : INFINITY;
+  needs = (!outside && !is_small && dc < load_fresh(&a.qmin[qid])) ? 1 : 0;
+}
+
+// The update for n <= 2 NT items (the default batch of 2048 on 1024
+// threads): item j = threadIdx + k NT stays in thread registers through all
+// phases, and the stable compaction scans slice k = 0, 1 in turn, so ranks
+// follow the item order as in cp_update_body.
+template <int NT>
+__device__ __forceinline__ void cp_update_regs(CpState* S, const CpArgs& a, int n, int pop) {
+  using Scan = rocprim::block_scan<int, NT>;
+  __shared__ typename Scan::storage_type scan_storage;
+  constexpr int KI = 2;
+  int qid[KI], needs[KI];
+  float lo[KI][3], up[KI][3], idist[KI];
+#pragma unroll
+  for (int k = 0; k < KI; ++k) {
+    const int j = threadIdx.x + k * NT;
+    needs[k] = 0; idist[k] = INFINITY; qid[k] = 0;
+    if (j < n) {
+      qid[k] = a.bq[j];
+      for (int d = 0; d < 3; ++d) { lo[k][d] = a.blo[3 * j + d]; up[k][d] = a.bup[3 * j + d]; }
+      cp_item(a, j, qid[k], lo[k], up[k], idist[k], needs[k]);
+    }
+  }
+  __syncthreads();   // every pre-round minimum is read before the scatter
+#pragma unroll
+  for (int k = 0; k < KI; ++k)
+    if (threadIdx.x + k * NT < n && idist[k] < INFINITY)
+      atomicMin(reinterpret_cast<int*>(&a.qmin[qid[k]]), __float_as_int(idist[k]));
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KI; ++k) {
+    const int j = threadIdx.x + k * NT;
+    if (j < n && idist[k] == load_fresh(&a.qmin[qid[k]])) atomicMax(&a.win[qid[k]], j);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KI; ++k) {
+    const int j = threadIdx.x + k * NT;
+    if (j < n && load_fresh(&a.win[qid[k]]) == j)
+      for (int d = 0; d < 3; ++d) a.qloc[3 * qid[k] + d] = 0.5f * (lo[k][d] + up[k][d]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KI; ++k)
+    if (threadIdx.x + k * NT < n) a.win[qid[k]] = -1;
+  int base = 0;
+#pragma unroll
+  for (int k = 0; k < KI; ++k) {
+    int r = 0, tot = 0;
+    Scan().exclusive_scan(needs[k], r, 0, tot, scan_storage);
+    if (needs[k]) cp_push_children(a, pop, base + r, qid[k], lo[k], up[k]);
+    base += tot;
+    __syncthreads();   // the scan storage is reused
+  }
+  if (threadIdx.x == 0) {
+    long long nt = (long long)pop + 2LL * base;
+    if (nt > a.cap) { S->overflow = 1; nt = a.cap; }
+    S->top = (int)nt;
+  }
+}
+
 // The round's update over the n popped items, by the NT threads of one
 // workgroup (src/kd_tree.py:692-760): distances, scatter-min, location, and
 // the stable compaction with the interleaved push at the pop position.
 template <int NT>
 __device__ __forceinline__ void cp_update_body(CpState* S, const CpArgs& a, int n, int pop) {
+  if (n <= 2 * NT) {
+    cp_update_regs<NT>(S, a, n, pop);
+    return;
+  }
   using Scan = rocprim::block_scan<int, NT>;
   __shared__ typename Scan::storage_type scan_storage;
   // phase 1: per-item distance and subdivision flag (pruning uses the minimum
   // from before this round's scatter)
   for (int j = threadIdx.x; j < n; j += NT) {
-    const int qid = a.bq[j];
-    float lo[3], up[3], c[3], e[3];
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = a.blo[3 * j + k]; up[k] = a.bup[3 * j + k];
-      c[k] = 0.5f * (lo[k] + up[k]); e[k] = up[k] - lo[k];
-    }
-    const float width = fmaxf(fmaxf(e[0], e[1]), e[2]);
-    const float diag = norm3(e[0], e[1], e[2]);
-    const float dc = norm3(a.q[3 * qid] - c[0], a.q[3 * qid + 1] - c[1], a.q[3 * qid + 2] - c[2]);
-    const float maxd = dc + diag;
-    const bool is_small = width < a.eps_cube_width;
-    const int code = a.codes[j];
-    const bool outside = code == CODE_NEGATIVE || code == CODE_POSITIVE;
-    bool all_neg = true, all_pos = true;
-    for (int t = 0; t < 7; ++t) {
-      const float v = a.vals[7 * j + t];
-      all_neg = all_neg && (v < 0.f);
-      all_pos = all_pos && (v > 0.f);
-    }
-    const bool spans = !(all_neg || all_pos);
-    a.idist[j] = spans ? maxd : INFINITY;
-    a.ineeds[j] = (!outside && !is_small && dc < load_fresh(&a.qmin[qid])) ? 1 : 0;
+    float lo[3], up[3];
+    for (int k = 0; k < 3; ++k) { lo[k] = a.blo[3 * j + k]; up[k] = a.bup[3 * j + k]; }
+    float idist;
+    int needs;
+    cp_item(a, j, a.bq[j], lo, up, idist, needs);
+    a.idist[j] = idist;
+    a.ineeds[j] = needs;
   }
   __syncthreads();
   // phase 2: scatter-min (non-negative floats order as their bit patterns)
@@ -151,22 +245,7 @@ __device__ __forceinline__ void cp_update_body(CpState* S, const CpArgs& a, int 
     if (!a.ineeds[j]) continue;
     float lo[3], up[3];
     for (int k = 0; k < 3; ++k) { lo[k] = a.blo[3 * j + k]; up[k] = a.bup[3 * j + k]; }
-    const float e0 = up[0] - lo[0], e1 = up[1] - lo[1], e2 = up[2] - lo[2];
-    int dim = 0; float em = e0;
-    if (e1 > em) { dim = 1; em = e1; }
-    if (e2 > em) dim = 2;
-    const float mid = 0.5f * (lo[dim] + up[dim]);
-    const long long pa = (long long)pop + 2LL * r, pb = pa + 1;
-    if (pb < a.cap) {
-      for (int k = 0; k < 3; ++k) {
-        a.wlo[3 * pa + k] = lo[k];
-        a.wup[3 * pa + k] = (k == dim) ? mid : up[k];
-        a.wlo[3 * pb + k] = (k == dim) ? mid : lo[k];
-        a.wup[3 * pb + k] = up[k];
-      }
-      a.wq[pa] = a.bq[j];
-      a.wq[pb] = a.bq[j];
-    }
+    cp_push_children(a, pop, r, a.bq[j], lo, up);
     ++r;
   }
   if (threadIdx.x == 0) {
