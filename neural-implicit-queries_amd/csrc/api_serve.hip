@@ -363,12 +363,19 @@ void Worker::run_tree_batch(std::vector<Request*>& group) {
   void* ws = nullptr;
   st = tree_batch_bytes(r0.model, &r0.cfg, &r0.tp, ng, &bytes);
   if (st == NIQ_OK && hipMallocAsync(&ws, bytes, s) != hipSuccess) st = fail(NIQ_ERR_HIP, "tree batch workspace");
-  if (st == NIQ_OK) st = tree_batch_build(models.data(), ng, &r0.cfg, &r0.tp, roots.data(), ws, bytes, res.data(), s);
+  std::vector<niq_status> tree_st(ng, NIQ_ERR_INVALID);
+  bool ran = false;
+  if (st == NIQ_OK) {
+    st = tree_batch_build(models.data(), ng, &r0.cfg, &r0.tp, roots.data(), ws, bytes, res.data(), tree_st.data(), s);
+    ran = st == NIQ_OK || st == NIQ_ERR_CAPACITY;
+  }
   if (ws) (void)hipFreeAsync(ws, s);
   if (hipStreamSynchronize(s) != hipSuccess && st == NIQ_OK) st = fail(NIQ_ERR_HIP, "hipStreamSynchronize failed");
-  for (Request* r : group) {
-    r->n_batched = ng;
-    finish(r, st);
+  for (int g = 0; g < ng; ++g) {
+    group[g]->n_batched = ng;
+    // a build that ran reports each tree's own outcome; otherwise every tree
+    // gets the batch's error
+    finish(group[g], ran && (st == NIQ_OK || st == NIQ_ERR_CAPACITY) ? tree_st[g] : st);
   }
 }
 

@@ -545,7 +545,7 @@ niq_status tree_batch_bytes(const niq_model* m, const niq_eval_cfg* cfg, const n
 
 niq_status tree_batch_build(const niq_model* const* models, int ng, const niq_eval_cfg* cfg,
                             const niq_tree_params* p, const float* roots, void* ws, size_t ws_bytes,
-                            niq_tree_result* const* res, hipStream_t s) {
+                            niq_tree_result* const* res, niq_status* tree_status, hipStream_t s) {
   if (ng < 1 || ng > kMaxTreeBatch) return fail(NIQ_ERR_INVALID, "tree batch size out of range");
   const int wt = models[0]->wt;
   for (int g = 0; g < ng; ++g) {
@@ -648,6 +648,8 @@ niq_status tree_batch_build(const niq_model* const* models, int ng, const niq_ev
   for (int g = 0; g < ng; ++g) {
     decode_counters(hc[g], n_levels, res[g]);
     overflow = overflow || hc[g].overflow;
+    // each tree fails as its lone build would: only when its own lists overflow
+    if (tree_status) tree_status[g] = hc[g].overflow ? NIQ_ERR_CAPACITY : NIQ_OK;
   }
   if (overflow) return fail(NIQ_ERR_CAPACITY, "a tree output buffer was too small");
   return NIQ_OK;

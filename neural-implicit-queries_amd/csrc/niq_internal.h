@@ -149,10 +149,12 @@ constexpr int kMaxTreeBatch = 64;
 bool tree_batchable(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p);
 niq_status tree_batch_bytes(const niq_model* m, const niq_eval_cfg* cfg, const niq_tree_params* p, int ng,
                             size_t* bytes);
-// roots: 6 floats per tree (lower then upper); synchronises.
+// roots: 6 floats per tree (lower then upper); tree_status (nullable): each
+// tree's own status once the build ran (NIQ_ERR_CAPACITY where its lists
+// overflowed); synchronises.
 niq_status tree_batch_build(const niq_model* const* models, int ng, const niq_eval_cfg* cfg,
                             const niq_tree_params* p, const float* roots, void* ws, size_t ws_bytes,
-                            niq_tree_result* const* res, hipStream_t s);
+                            niq_tree_result* const* res, niq_status* tree_status, hipStream_t s);
 niq_status classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, int nvec, const ClassifyArgs& a,
                              long long max_n, hipStream_t s);
 // Whether a classification of `nvec` box vectors keeps its planes in global

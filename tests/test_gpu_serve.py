@@ -288,3 +288,35 @@ def test_graph_not_replayed_for_a_new_model_at_a_freed_address():
         res = got.result()
         assert not got.info["graph_reused"]
     assert same_tree(alone, res)
+
+
+def test_batched_tree_overflow_fails_only_its_tree():
+    """In a batched build, a tree whose output list is too small fails with
+    NIQ_ERR_CAPACITY, as its lone build would; the other trees of the batch
+    finish and equal their lone builds."""
+    import ctypes
+    note("[serve] overflow inside a batched build")
+    (f1, p1), (f2, p2), _ = setup()
+    kw = dict(split_depth=11, with_interior_nodes=True, with_exterior_nodes=True)
+    alone = [kd_tree.construct_uniform_unknown_levelset_tree(f, p, LO, UP, **kw) for f, p in ((f1, p1), (f2, p2))]
+    lib = _lib.load_library()
+    with serve.Server(n_streams=1) as srv:
+        busy = srv.submit_tree(f1, p1, LO, UP, split_depth=16)
+        good1 = srv.submit_tree(f1, p1, LO, UP, **kw)
+        # the second model's build with a one-row unknown-node list
+        m, p, cap, cap_fin = kd_tree.tree_request(f2, p2, LO, UP, **kw)
+        srv._models.append(m)
+        bufs = kd_tree.TreeBuffers(p, 1, cap_fin, torch.device("cuda"))
+        srv._inputs_ready()
+        rid = ctypes.c_uint64()
+        _lib.check(lib.niq_serve_submit_tree(srv._h, m.handle, ctypes.byref(f2.cfg), ctypes.byref(p),
+                                             ctypes.byref(bufs.r), ctypes.byref(rid)))
+        bad = serve.Request(srv, rid.value, "tree", {"bufs": bufs, "tree_res": bufs.r, "cfg": f2.cfg, "p": p},
+                            lambda r: r._keep["bufs"].outputs()[0])
+        good2 = srv.submit_tree(f2, p2, LO, UP, **kw)
+        busy.result()
+        assert same_tree(alone[0], good1.result()) and same_tree(alone[1], good2.result())
+        with pytest.raises(_lib.CapacityError):
+            bad.result()
+    note(f"  batch sizes {good1.info['n_batched']}, {bad.info['n_batched']}, {good2.info['n_batched']}")
+    assert good1.info["n_batched"] == 3 and bad.info["n_batched"] == 3 and good2.info["n_batched"] == 3
