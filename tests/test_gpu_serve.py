@@ -170,12 +170,15 @@ def test_mix_equals_standalone():
     note(f"  {replays} of {len(t_trees)} trees from graphs, {batched} in batched builds; latency {lat}")
 
 
-def test_tree_requests_merge_into_one_build():
+@pytest.mark.parametrize("mode", ["affine_fixed", "interval", "slope_interval"])
+def test_tree_requests_merge_into_one_build(mode):
     """Tree requests waiting behind a running one, with different models and
     root boxes but one config, depth and template width, run as one batched
     build; each result equals its lone build bit for bit."""
-    note("[serve] batched tree builds")
-    (f1, p1), (f2, p2), _ = setup()
+    note(f"[serve] batched tree builds, {mode}")
+    (_, p1), (_, p2), _ = setup()
+    f1, _ = implicit_mlp_utils.implicit_from_params(p1, mode)
+    f2, _ = implicit_mlp_utils.implicit_from_params(p2, mode)
     assert max(nmlp.device_model(p1).dims) <= 64 and max(nmlp.device_model(p2).dims) <= 64   # one template width
     boxes = [(LO, UP), (np.array([-1, -1, -1], np.float32), np.array([0, 0, 0], np.float32)),
              (np.array([-0.5, -0.25, -1], np.float32), np.array([0.5, 0.75, 0], np.float32))]
