@@ -66,8 +66,9 @@ typedef struct niq_eval_cfg {
   int n_append;        /* affine_append: rows appended per activation */
   float recheck_tol;   /* > 0: FP64 re-test of cells within tol*scale of the threshold */
   int variant;         /* kernel variant, 0 = default; plain FP32: 1 TileCfg,
-                          2 TileCfgAlt, 3 TileCfg with 256 registers
-                          (niq_tile_shape); growing-row modes: 2 half the
+                          2 TileCfgAlt, 3 TileCfg with 256 registers, 4
+                          TileCfgWide (the most waves, the narrowest width
+                          tile per wave) (niq_tile_shape); growing-row modes: 2 half the
                           error-term rows per chunk, 3 a 2-waves-per-SIMD
                           register budget; precision mode: 1 streams the
                           packed bf16 weight copies, 2 splits FP32 weights in
@@ -111,7 +112,7 @@ niq_status niq_classify_general_boxes(const niq_model* m, const niq_eval_cfg* cf
                                       float* bounds, void* stream);
 
 /* Tile shape of a kernel variant, for autotuners and reports: which 0 is the
- * plain FP32 classify pass (eval cfg variant 0-3), 1 the ray caster
+ * plain FP32 classify pass (eval cfg variant 0-4), 1 the ray caster
  * (ray_variant 0-2), 2 the growing-row classify (variant 0-3); width a layer
  * width of the model (its template width is used).  *nb: boxes or ray slots
  * per workgroup (which 2: error-term rows per chunk; one box per workgroup),

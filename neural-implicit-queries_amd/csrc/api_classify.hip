@@ -48,8 +48,11 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
 }
 
 // Tile variants for the forms trees use most (interval and affine_fixed over
-// 3 box vectors): 1 (and 0) TileCfg, 2 TileCfgAlt, 3 TileCfgRegs; the other
-// forms always take TileCfg.
+// 3 box vectors): 1 (and 0) TileCfg, 2 TileCfgAlt, 3 TileCfgRegs, 4
+// TileCfgWide (16 boxes, the most waves: the narrowest width tile per wave);
+// the other forms always take TileCfg.  The waves of a tile split the layer's
+// feature tiles among them, so the wave count sets the width tile each wave
+// contracts (W / 16 / NW tiles of 16 features).
 template <int KIND, int V>
 niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s,
                            int variant = 0) {
@@ -81,6 +84,15 @@ niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long 
         case 128: return launch_classify_w<KIND, V, 128, TileCfgRegs<128>>(m, a, max_n, s);
         case 256: return launch_classify_w<KIND, V, 256, TileCfgRegs<256>>(m, a, max_n, s);
         case 512: return launch_classify_w<KIND, V, 512, TileCfgRegs<512>>(m, a, max_n, s);
+      }
+      return fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+    if (variant == 4) {   // the narrowest width tile: one 16-feature tile per wave (at W <= 256)
+      switch (m->wt) {
+        case 64: return launch_classify_w<KIND, V, 64, TileCfgWide<64>>(m, a, max_n, s);
+        case 128: return launch_classify_w<KIND, V, 128, TileCfgWide<128>>(m, a, max_n, s);
+        case 256: return launch_classify_w<KIND, V, 256, TileCfgWide<256>>(m, a, max_n, s);
+        case 512: return launch_classify_w<KIND, V, 512, TileCfgWide<512>>(m, a, max_n, s);
       }
       return fail(NIQ_ERR_UNSUPPORTED, "width");
     }
@@ -220,6 +232,7 @@ niq_status tile_shape_w(int which, int variant, int* nb, int* nw, int* occ) {
       case 0: case 1: shape_of<TileCfg<W>, W>(5, nb, nw, occ); return NIQ_OK;
       case 2: shape_of<TileCfgAlt<W>, W>(5, nb, nw, occ); return NIQ_OK;
       case 3: shape_of<TileCfgRegs<W>, W>(5, nb, nw, occ); return NIQ_OK;
+      case 4: shape_of<TileCfgWide<W>, W>(5, nb, nw, occ); return NIQ_OK;
     }
   } else if (which == 2) {   // growing-row classify: *nb is the error-term rows per chunk
     return rows_tile_shape(variant, W, nb, nw, occ);

@@ -99,7 +99,7 @@ static void rows_shape_w(int variant, int* ch, int* nw, int* occ) {
 }
 
 niq_status rows_tile_shape(int variant, int wt, int* ch, int* nw, int* occ) {
-  if (variant < 0 || variant > 3) return fail(NIQ_ERR_INVALID, "unknown variant");
+  if (variant < 0 || variant > 4) return fail(NIQ_ERR_INVALID, "unknown variant");   // 4 runs the default
   switch (wt) {
     case 64: rows_shape_w<64>(variant, ch, nw, occ); return NIQ_OK;
     case 128: rows_shape_w<128>(variant, ch, nw, occ); return NIQ_OK;

@@ -77,11 +77,16 @@ bool same_cast(const Request& a, const Request& b) {
          req_wt(a, false) == req_wt(b, false) && req_wt(a, true) == req_wt(b, true);
 }
 
-// Tree requests that can share one batched build: the same eval config and
-// build params (root boxes aside), one template width, a batchable form.
+// Tree requests that can share one batched build: the same eval config (its
+// tile variants aside: they change no result, and a batch runs the default
+// tile) and build params (root boxes aside), one template width, a batchable
+// form.
 bool same_tree_batch(const Request& a, const Request& b) {
   if (a.kind != REQ_TREE || b.kind != REQ_TREE) return false;
-  if (std::memcmp(&a.cfg, &b.cfg, sizeof(niq_eval_cfg)) != 0 || a.model->wt != b.model->wt) return false;
+  niq_eval_cfg ca = a.cfg, cb = b.cfg;
+  ca.variant = cb.variant = 0;
+  ca.ray_variant = cb.ray_variant = 0;
+  if (std::memcmp(&ca, &cb, sizeof(niq_eval_cfg)) != 0 || a.model->wt != b.model->wt) return false;
   const niq_tree_params &p = a.tp, &q = b.tp;
   return p.split_depth == q.split_depth && p.node_terminate_thresh == q.node_terminate_thresh &&
          p.batch_process_size == q.batch_process_size && p.with_interior_nodes == q.with_interior_nodes &&

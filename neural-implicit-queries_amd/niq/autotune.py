@@ -7,7 +7,8 @@ The search space is the kernels' tile shapes, each a compiled variant:
   workgroup, 4-8 waves, the register budget LDS occupancy allows), 2 twice
   the boxes per workgroup where LDS allows (fewer waves at width 512), 3 the
   default tile with a 256-register budget (no spills, fewer resident
-  workgroups);
+  workgroups), 4 16 boxes with the most waves (the narrowest width tile: the
+  waves of a tile split the layer's 16-feature tiles among them);
 * precision mode (BF16) classify: 1 streams the packed bf16 weight copies,
   2 loads the FP32 weights once and splits them in registers, 3 runs layer
   by layer (one batched launch per hidden layer, activations through HBM);
@@ -39,24 +40,25 @@ import torch
 from . import _lib
 from . import mlp as _mlp
 
-VARIANTS = (1, 2, 3)
+VARIANTS = (1, 2, 3, 4)
+VARIANTS_ROWS = (1, 2, 3)
 VARIANTS_BF16 = (1, 2, 3)
 RAY_VARIANTS = (0, 1, 2)
 _cache = {}
 _ray_cache = {}
+ROWS_MODES = (_lib.MODE["affine_truncate"], _lib.MODE["affine_append"], _lib.MODE["affine_all"])
 
 
 def variants_of(func):
-    return VARIANTS_BF16 if int(func.cfg.precision) == 1 else VARIANTS
+    if int(func.cfg.precision) == 1:
+        return VARIANTS_BF16
+    return VARIANTS_ROWS if int(func.cfg.mode) in ROWS_MODES else VARIANTS
 
 
 def _key(func, params):
     m = _mlp.device_model(params)
     acts = tuple(L.act for L in m.layers)
     return (tuple(m.dims), acts, int(func.cfg.mode), int(func.cfg.precision))
-
-
-ROWS_MODES = (_lib.MODE["affine_truncate"], _lib.MODE["affine_append"], _lib.MODE["affine_all"])
 
 
 def tile_shape(kernel, variant, width):
