@@ -21,6 +21,7 @@
 // forms (interval, affine_fixed) run here.
 #pragma once
 #include "niq_device.h"
+#include "niq_kernels.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -151,7 +152,26 @@ template <int V> struct PlanesBf16 {
   static constexpr int E = 3 + 2 * V;       // index of the e plane
 };
 
+// Waves per SIMD that a tile's bf16 planes admit (workgroups per CU by LDS x
+// waves per workgroup / 4 SIMDs), capped at 4: the fused kernel's register
+// budget, so the compiler does not trade registers for occupancy LDS
+// already rules out (at width 512 one workgroup per CU: 256 VGPRs a wave).
+template <int V, int NB, int NW, int WMAX>
+struct BfOcc {
+  static constexpr int wgs = (int)((size_t)kLdsLimit / ((size_t)PlanesBf16<V>::NPL * NB * WMAX * 2));
+  static constexpr int raw = wgs * NW / 4;
+  static constexpr int value = raw < 1 ? 1 : (raw > 4 ? 4 : raw);
+};
+
 constexpr float kU24 = 5.9604644775390625e-08f;   // 2^-24
+
+// A global-address-space view of a device pointer: loads through it are
+// global loads, counted by vmcnt only (flat loads also count against
+// lgkmcnt, so an LDS wait would wait for them as well).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
 
 // Split and store one (box, feature) element's affine form: base three ways,
 // aff two ways, and the e plane err + c_base |base| + c_aff sum |aff|.
@@ -308,15 +328,18 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
       constexpr bool FULL = decltype(full_tag)::value;
       floatx4 wpf[FTMAX][2];
       bf16x8 wnx[FTMAX][4];
+      // global loads (vmcnt only), so the LDS wait before a chunk's first
+      // MFMA does not wait for the next chunk's weights as well
       auto fetch = [&](int f, int kb) {
         const int ft = wave + f * NW;
         if constexpr (WS == WSRC_FP32) {
-          const floatx4* P = Ly.W32 + 2 * (size_t)(ft * nkb + kb) * 64 + lane;
+          const auto P = as_global(Ly.W32) + 2 * (size_t)(ft * nkb + kb) * 64 + lane;
           wpf[f][0] = P[0];
           wpf[f][1] = P[64];
         } else {
           const size_t o = (size_t)(ft * nkb + kb) * 64 + lane;
-          wnx[f][0] = Ly.W1[o]; wnx[f][1] = Ly.W2[o]; wnx[f][2] = Ly.W3[o]; wnx[f][3] = Ly.Wa[o];
+          wnx[f][0] = as_global(Ly.W1)[o]; wnx[f][1] = as_global(Ly.W2)[o];
+          wnx[f][2] = as_global(Ly.W3)[o]; wnx[f][3] = as_global(Ly.Wa)[o];
         }
       };
 #pragma unroll
@@ -347,8 +370,10 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
             } else {
               w1 = wnx[f][0]; w2 = wnx[f][1]; w3 = wnx[f][2]; wa1 = wnx[f][3];
             }
-            // the next chunk's fragment goes into the same registers
-            if (t + 1 == BT && kb + 1 < nkb) fetch(f, kb + 1);
+            // the next chunk's fragment goes into the same registers; the
+            // last chunk fetches itself again, so the loop body has no branch
+            // and the load waits count only what is needed
+            if (t + 1 == BT) fetch(f, kb + 1 < nkb ? kb + 1 : kb);
             // base row, smallest products first, fresh accumulator per chunk
             {
               floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, x3, z4, 0, 0, 0);
@@ -398,6 +423,10 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
             }
           }
         }
+        // keeps the next chunk's weight loads in this iteration (issued one
+        // chunk ahead); otherwise they may be moved to the top of the next
+        // iteration, just before their use
+        __asm__ volatile("" ::: "memory");
       }
     };
     if (wave + (FTMAX - 1) * NW < nft) contract(std::true_type{});
@@ -480,7 +509,7 @@ __device__ __forceinline__ void final_bounds_fp64(const float* v, double& lo, do
 }
 
 template <int V, int NB, int NW, int WMAX, int WS>
-__global__ void __launch_bounds__(NW * 64)
+__global__ void __launch_bounds__(NW * 64, (BfOcc<V, NB, NW, WMAX>::value))
 k_classify_bf16(const DevModelBf16* __restrict__ Mp, ClassifyArgs a) {
   using PB = PlanesBf16<V>;
   constexpr int NT = NW * 64;
