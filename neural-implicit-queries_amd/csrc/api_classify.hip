@@ -34,7 +34,7 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
     auto k = k_classify<KIND, V, NB, NW, W, false, OCC>;
     const long long tiles = (max_n + NB - 1) / NB;
     if (tiles == 0) return NIQ_OK;
-    const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+    const unsigned g = tile_grid(tiles, persistent_blocks(k, NW * 64));
     hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, a);
     HIPCHK(hipGetLastError());
     if (a.recheck_tol > 0.f) {
@@ -113,7 +113,7 @@ niq_status launch_classify_sdf_w(const niq_model* m, const ClassifyArgs& a, long
   auto k = k_classify_sdf<NB, NW, W>;
   const long long tiles = (max_n + NB - 1) / NB;
   if (tiles == 0) return NIQ_OK;
-  const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+  const unsigned g = tile_grid(tiles, persistent_blocks(k, NW * 64));
   hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev, a);
   HIPCHK(hipGetLastError());
   return NIQ_OK;

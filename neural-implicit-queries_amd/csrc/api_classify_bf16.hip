@@ -25,7 +25,7 @@ niq_status launch_bf16_w(const niq_model* m, const ClassifyArgs& a, long long ma
     auto k = k_classify_bf16<V, NB, NW, W, WS>;
     const long long tiles = (max_n + NB - 1) / NB;
     if (tiles == 0) return NIQ_OK;
-    const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
+    const unsigned g = tile_grid(tiles, persistent_blocks(k, NW * 64));
     hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, m->dev16, a);
     HIPCHK(hipGetLastError());
     return NIQ_OK;

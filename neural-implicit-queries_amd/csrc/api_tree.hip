@@ -296,7 +296,7 @@ niq_status launch_classify_items_w(const TreeItem* items, int ng, const TreeOffs
     hipLaunchKernelGGL(k_items_prefix, dim3(1), dim3(64), 0, s, items, ng, o, level, NB, pref);
     auto k = k_classify_items<KIND, V, NB, NW, W>;
     const long long max_tiles = (long long)ng * ((max_n + NB - 1) / NB);
-    const unsigned g = grid_for(max_tiles, persistent_blocks(k, NW * 64));
+    const unsigned g = tile_grid(max_tiles, persistent_blocks(k, NW * 64));
     hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, items, ng, o, level, offset, pref);
     HIPCHK(hipGetLastError());
     return NIQ_OK;

@@ -107,6 +107,14 @@ inline unsigned grid_for(long long work_blocks, long long persist) {
   return (unsigned)std::max(1LL, g);
 }
 
+// Grid of a classification launch (each workgroup strides over box tiles):
+// up to kTileGridMult workgroups per resident slot.  Its workgroups then
+// retire while the launch runs, so launches on other streams (a concurrent
+// tree's count, scan and scatter kernels) get compute units between them
+// instead of waiting for the whole launch to drain.
+constexpr long long kTileGridMult = 8;
+inline unsigned tile_grid(long long tiles, long long persist) { return grid_for(tiles, persist * kTileGridMult); }
+
 }  // namespace niqi
 
 using namespace niqi;
