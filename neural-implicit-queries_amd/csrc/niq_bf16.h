@@ -165,14 +165,6 @@ struct BfOcc {
 
 constexpr float kU24 = 5.9604644775390625e-08f;   // 2^-24
 
-// A global-address-space view of a device pointer: loads through it are
-// global loads, counted by vmcnt only (flat loads also count against
-// lgkmcnt, so an LDS wait would wait for them as well).
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
-  return (const __attribute__((address_space(1))) T*)p;
-}
-
 // Split and store one (box, feature) element's affine form: base three ways,
 // aff two ways, and the e plane err + c_base |base| + c_aff sum |aff|.
 template <int V, int NB, int WMAX>

@@ -48,6 +48,14 @@ struct DevModel {
 // ------------------------------------------------------------------ helpers
 // Math by element type, so each rule is written once for FP32 (the kernels)
 // and FP64 (the near-threshold re-test).
+// A global-address-space view of a device pointer: loads through it are
+// global loads, counted by vmcnt only (flat loads also count against
+// lgkmcnt, so an LDS wait would wait for them as well).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
 template <class T> struct Fm;
 template <> struct Fm<float> {
   static __device__ __forceinline__ float exp(float x) { return expf(x); }

@@ -25,7 +25,7 @@ enum { ROWS_ALL = 0, ROWS_TRUNCATE = 1, ROWS_APPEND = 2 };
 
 template <int WMAX> struct RowsCfg;
 template <> struct RowsCfg<64>  { static constexpr int CH = 64, NW = 4; };
-template <> struct RowsCfg<128> { static constexpr int CH = 64, NW = 4; };
+template <> struct RowsCfg<128> { static constexpr int CH = 32, NW = 4; };
 template <> struct RowsCfg<256> { static constexpr int CH = 32, NW = 4; };
 template <> struct RowsCfg<512> { static constexpr int CH = 32, NW = 4; };
 
@@ -65,8 +65,8 @@ __device__ __forceinline__ int block_argmax(const float* vals, const unsigned ch
   return r;
 }
 
-// Register budget: the waves per SIMD that LDS admits (four 41 KB workgroups
-// per CU at width <= 128).
+// Register budget: four waves per SIMD at width <= 128 (LDS admits four
+// workgroups per CU or more there).
 template <int WMAX> struct RowsOcc { static constexpr int value = WMAX <= 128 ? 4 : 1; };
 
 // OCC: the register budget in waves per SIMD (RowsOcc by default; 2 trades
@@ -193,8 +193,9 @@ k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
           const int e = 4 * ((int)threadIdx.x + j * NT);
           const int rr = e / Ly.in_pad, k = e % Ly.in_pad;
           const long long r = r0 + rr;
-          ld[j] = (e < CH * Ly.in_pad && r < n_stored) ? *reinterpret_cast<const floatx4*>(&Ein[r * WMAX + k])
-                                                       : floatx4{0.f, 0.f, 0.f, 0.f};
+          ld[j] = (e < CH * Ly.in_pad && r < n_stored)
+                      ? as_global(reinterpret_cast<const floatx4*>(Ein))[(r * WMAX + k) >> 2]
+                      : floatx4{0.f, 0.f, 0.f, 0.f};
         }
         __syncthreads();
 #pragma unroll
@@ -213,38 +214,50 @@ k_classify_rows(const DevModel* __restrict__ Mp, RowsArgs ra) {
         for (int f = 0; f < FTMAX; ++f)
 #pragma unroll
           for (int t = 0; t < BT; ++t) acc[f][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-        // weights one k-block ahead of the MFMAs
-        floatx4 w_nx[FTMAX];
+        // weights one k-block ahead of the MFMAs.  `FULL`: every feature
+        // tile of this wave exists, so the loop body carries no per-tile
+        // test and is one block the scheduler can order freely
+        const auto Wp = as_global(Ly.Wp);
+        auto kloop = [&](auto full_tag) {
+          constexpr bool FULL = decltype(full_tag)::value;
+          floatx4 w_nx[FTMAX];
 #pragma unroll
-        for (int f = 0; f < FTMAX; ++f) {
-          const int ft = wave + f * NW;
-          if (ft < nft) w_nx[f] = Ly.Wp[(size_t)(ft * nkb) * 64 + lane];
-        }
-        for (int kb = 0; kb < nkb; ++kb) {
-          floatx4 w[FTMAX];
-#pragma unroll
-          for (int f = 0; f < FTMAX; ++f) w[f] = w_nx[f];
-          if (kb + 1 < nkb) {
-#pragma unroll
-            for (int f = 0; f < FTMAX; ++f) {
-              const int ft = wave + f * NW;
-              if (ft < nft) w_nx[f] = Ly.Wp[(size_t)(ft * nkb + kb + 1) * 64 + lane];
-            }
+          for (int f = 0; f < FTMAX; ++f) {
+            const int ft = wave + f * NW;
+            if (FULL || ft < nft) w_nx[f] = Wp[(size_t)(ft * nkb) * 64 + lane];
           }
+          for (int kb = 0; kb < nkb; ++kb) {
+            floatx4 w[FTMAX];
 #pragma unroll
-          for (int t = 0; t < BT; ++t) {
-            const floatx4 bx = *reinterpret_cast<const floatx4*>(&X[xoff<CH, WMAX>(0, t * 16 + ln, kb * 16 + lg * 4)]);
+            for (int f = 0; f < FTMAX; ++f) w[f] = w_nx[f];
+            // the last k-block fetches itself again: no branch in the loop
+            const int kn = kb + 1 < nkb ? kb + 1 : kb;
 #pragma unroll
             for (int f = 0; f < FTMAX; ++f) {
               const int ft = wave + f * NW;
-              if (ft < nft) {
+              if (FULL || ft < nft) w_nx[f] = Wp[(size_t)(ft * nkb + kn) * 64 + lane];
+            }
+            // issue point of those loads: ahead of this k-block's MFMAs (their
+            // wait comes with the copy at the next k-block)
+            __asm__ volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                  acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[f][j], bx[j], acc[f][t], 0, 0, 0);
+            for (int t = 0; t < BT; ++t) {
+              const floatx4 bx = *reinterpret_cast<const floatx4*>(&X[xoff<CH, WMAX>(0, t * 16 + ln, kb * 16 + lg * 4)]);
+#pragma unroll
+              for (int f = 0; f < FTMAX; ++f) {
+                const int ft = wave + f * NW;
+                if (FULL || ft < nft) {
+#pragma unroll
+                  for (int j = 0; j < 4; ++j)
+                    acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[f][j], bx[j], acc[f][t], 0, 0, 0);
+                }
               }
             }
           }
-        }
+        };
+        if (wave + (FTMAX - 1) * NW < nft) kloop(std::true_type{});
+        else kloop(std::false_type{});
 #pragma unroll
         for (int f = 0; f < FTMAX; ++f) {
           const int ft = wave + f * NW;
