@@ -4,8 +4,9 @@
 // ------------------------------------------------------------- dispatch
 namespace {
 
-// Launch k_classify for (KIND, V) at template width W with tile T.
-template <int KIND, int V, int W, class T = TileCfg<W>>
+// Launch k_classify for (KIND, V) at template width W with tile T, compiled
+// for the activation set AS.
+template <int KIND, int V, int W, class T = TileCfg<W>, int AS = ACTS_ALL>
 niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
   constexpr int NB = T::NB, NW = T::NW;
   constexpr int OCC = T::OCC ? std::min(T::OCC, LdsOcc<Planes<KIND, V, 0>::R, NB, NW, W>::value) : 0;
@@ -31,7 +32,7 @@ niq_status launch_classify_w(const niq_model* m, const ClassifyArgs& a, long lon
     }
     return NIQ_OK;
   } else {
-    auto k = k_classify<KIND, V, NB, NW, W, false, OCC>;
+    auto k = k_classify<KIND, V, NB, NW, W, false, OCC, AS>;
     const long long tiles = (max_n + NB - 1) / NB;
     if (tiles == 0) return NIQ_OK;
     const unsigned g = tile_grid(tiles, persistent_blocks(k, NW * 64));
@@ -93,6 +94,20 @@ niq_status launch_classify(const niq_model* m, const ClassifyArgs& a, long long 
         case 128: return launch_classify_w<KIND, V, 128, TileCfgWide<128>>(m, a, max_n, s);
         case 256: return launch_classify_w<KIND, V, 256, TileCfgWide<256>>(m, a, max_n, s);
         case 512: return launch_classify_w<KIND, V, 512, TileCfgWide<512>>(m, a, max_n, s);
+      }
+      return fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+  }
+  if constexpr (KIND == 0 && (V == 0 || V == 3)) {
+    // ReLU models, in the forms trees use (the default tile): a kernel
+    // compiled for ReLU alone, the same sums and rules (bit-identical), fewer
+    // registers
+    if (m->relu_only) {
+      switch (m->wt) {
+        case 64: return launch_classify_w<KIND, V, 64, TileCfg<64>, ACTS_RELU>(m, a, max_n, s);
+        case 128: return launch_classify_w<KIND, V, 128, TileCfg<128>, ACTS_RELU>(m, a, max_n, s);
+        case 256: return launch_classify_w<KIND, V, 256, TileCfg<256>, ACTS_RELU>(m, a, max_n, s);
+        case 512: return launch_classify_w<KIND, V, 512, TileCfg<512>, ACTS_RELU>(m, a, max_n, s);
       }
       return fail(NIQ_ERR_UNSUPPORTED, "width");
     }

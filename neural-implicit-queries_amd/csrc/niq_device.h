@@ -362,8 +362,18 @@ __device__ __forceinline__ void final_bounds(const T* v, T& lo, T& hi, T& scale)
   }
 }
 
-template <class PL, int KIND, int V, class T>
+// Activation sets a kernel is compiled for: every activation, or ReLU alone
+// (models whose hidden layers are all ReLU; the rules of the others and their
+// constants then take no registers).
+enum { ACTS_ALL = 0, ACTS_RELU = 1 };
+
+template <class PL, int KIND, int V, int AS = ACTS_ALL, class T>
 __device__ __forceinline__ void apply_hidden(int act, T* v) {
+  if constexpr (AS == ACTS_RELU) {
+    // the same rule with the activation a constant (identity otherwise)
+    if (act == ACT_RELU) apply_hidden<PL, KIND, V, ACTS_ALL>(ACT_RELU, v);
+    return;
+  }
   if constexpr (KIND == 0) affine_fixed_rule<V>(act, v);
   else if constexpr (KIND == 1) slope_rule<V>(act, v);
   else if constexpr (KIND == 3) {
@@ -388,7 +398,7 @@ __device__ __forceinline__ int xoff(int p, int b, int k) {
   return (p * NB + b) * WMAX + (k ^ ((b & 7) << 2));
 }
 
-template <class PL, int KIND, int V, int NB, int NW, int WMAX, class Fin>
+template <class PL, int KIND, int V, int NB, int NW, int WMAX, int AS = ACTS_ALL, class Fin>
 __device__ __forceinline__ void mlp_forward(const DevModel& M, float* __restrict__ X, Fin&& fin) {
   constexpr int R = PL::R;
   constexpr int BT = NB / 16;
@@ -539,7 +549,7 @@ __device__ __forceinline__ void mlp_forward(const DevModel& M, float* __restrict
 #pragma unroll
           for (int p = 0; p < R; ++p) v[p] = acc[f][p][t][j] + (PL::bias(p) ? bias4[j] : 0.f);
           if (!last) {
-            apply_hidden<PL, KIND, V>(Ly.act, v);
+            apply_hidden<PL, KIND, V, AS>(Ly.act, v);
           } else if (ft == 0 && lg == 0 && j == 0) {
             fin(v, b);
           }

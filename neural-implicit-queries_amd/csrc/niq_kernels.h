@@ -135,7 +135,7 @@ __device__ __forceinline__ void fill_box_inputs(float* X, const ClassifyArgs& a,
   }
 }
 
-template <int KIND, int V, int NB, int NW, int WMAX, bool GX = false, int OCC = 0>
+template <int KIND, int V, int NB, int NW, int WMAX, bool GX = false, int OCC = 0, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64, (OCC ? OCC : (GX ? 2 : LdsOcc<Planes<KIND, V, 0>::R, NB, NW, WMAX>::value)))
 k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
   using PL = Planes<KIND, V, 0>;
@@ -153,7 +153,7 @@ k_classify(const DevModel* __restrict__ Mp, ClassifyArgs a) {
   for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const long long i0 = tile * NB;
     fill_box_inputs<KIND, V, NB, NT, WMAX>(X, a, i0, n);
-    mlp_forward<PL, KIND, V, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+    mlp_forward<PL, KIND, V, NB, NW, WMAX, AS>(*Mp, X, [&](const float* v, int b) {
       const long long i = i0 + b;
       if (i >= n) return;
       float lo, hi, scale;
