@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes of one 262,144-box classify launch of the 3->256x6->1 model:
 # rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by what the counters read for a
 # plain 1 GiB copy (profiles/k_classify_c2_relu256x6_tree18.md)
-C2_CLASSIFY_TRAFFIC_BYTES = 24813378 + 21947482
+C2_CLASSIFY_TRAFFIC_BYTES = 19309870 + 1048562
 
 
 def progress(msg):
