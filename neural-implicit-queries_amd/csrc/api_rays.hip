@@ -17,7 +17,7 @@ size_t niq_cast_workspace_bytes(const niq_cast_opts* opts) {
 
 namespace {
 
-template <int KIND, int V, int W, int VAR>
+template <int KIND, int V, int W, int VAR, int AS = ACTS_ALL>
 niq_status launch_rays_w(const DevModel* models, const RayArgs& a, long long n_rays, hipStream_t s) {
   constexpr int NB = RayTile<W, VAR>::NB, NW = RayTile<W, VAR>::NW;
   constexpr int R = Planes<KIND, V, KIND == 2 ? 3 : 2>::R;
@@ -26,7 +26,7 @@ niq_status launch_rays_w(const DevModel* models, const RayArgs& a, long long n_r
   if constexpr (!Fits<R, NB, W>::value) {
     return fail(NIQ_ERR_UNSUPPORTED, "ray range form does not fit in LDS at this width");
   } else {
-    auto k = k_cast_rays<KIND, V, NB, NW, W, OCC>;
+    auto k = k_cast_rays<KIND, V, NB, NW, W, OCC, AS>;
     const long long tiles = (n_rays + NB - 1) / NB;
     const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
     hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, models, a);
@@ -35,20 +35,26 @@ niq_status launch_rays_w(const DevModel* models, const RayArgs& a, long long n_r
   }
 }
 
-template <int KIND, int V, int VAR>
+template <int KIND, int V, int VAR, int AS = ACTS_ALL>
 niq_status launch_rays_v(int wt, const DevModel* models, const RayArgs& a, long long n_rays, hipStream_t s) {
   switch (wt) {
-    case 64: return launch_rays_w<KIND, V, 64, VAR>(models, a, n_rays, s);
-    case 128: return launch_rays_w<KIND, V, 128, VAR>(models, a, n_rays, s);
-    case 256: return launch_rays_w<KIND, V, 256, VAR>(models, a, n_rays, s);
-    case 512: return launch_rays_w<KIND, V, 512, VAR>(models, a, n_rays, s);
+    case 64: return launch_rays_w<KIND, V, 64, VAR, AS>(models, a, n_rays, s);
+    case 128: return launch_rays_w<KIND, V, 128, VAR, AS>(models, a, n_rays, s);
+    case 256: return launch_rays_w<KIND, V, 256, VAR, AS>(models, a, n_rays, s);
+    case 512: return launch_rays_w<KIND, V, 512, VAR, AS>(models, a, n_rays, s);
   }
   return fail(NIQ_ERR_UNSUPPORTED, "width");
 }
 
+// relu_only: every model of the cast has only ReLU hidden layers; the default
+// affine_fixed tile then runs the kernel compiled for ReLU alone
+// (bit-identical, fewer registers).
 template <int KIND, int V>
-niq_status launch_rays(int wt, int variant, const DevModel* models, const RayArgs& a, long long n_rays,
-                       hipStream_t s) {
+niq_status launch_rays(int wt, int variant, bool relu_only, const DevModel* models, const RayArgs& a,
+                       long long n_rays, hipStream_t s) {
+  if constexpr (KIND == 0 && V == 1) {
+    if (variant == 0 && relu_only) return launch_rays_v<KIND, V, 0, ACTS_RELU>(wt, models, a, n_rays, s);
+  }
   switch (variant) {
     case 0: return launch_rays_v<KIND, V, 0>(wt, models, a, n_rays, s);
     case 1: return launch_rays_v<KIND, V, 1>(wt, models, a, n_rays, s);
@@ -218,9 +224,11 @@ niq_status cast_run_groups(CastGroupReq* reqs, int ng, const niq_eval_cfg* cfg, 
   }
   long long n_rays = 0;
   int m0 = 0;
+  bool relu = true;
   for (int g = 0; g < ng; ++g) {
     const CastGroupReq& r = reqs[g];
     for (int f = 0; f < r.n_models; ++f) {
+      relu = relu && r.models[f]->relu_only;
       ht[m0 + f] = r.models[f]->host;
       if (bf16) ht16[m0 + f] = r.models[f]->host16;
     }
@@ -258,10 +266,10 @@ niq_status cast_run_groups(CastGroupReq* reqs, int ng, const niq_eval_cfg* cfg, 
     a.n_groups = ng;
     if (bf16) st = mode == NIQ_MODE_AFFINE_FIXED ? launch_rays_bf16<1>(wt16, table16, a, n_rays, s)
                                                  : launch_rays_bf16<0>(wt16, table16, a, n_rays, s);
-    else if (mode == NIQ_MODE_AFFINE_FIXED) st = launch_rays<0, 1>(wt, rv, table, a, n_rays, s);
-    else if (mode == NIQ_MODE_INTERVAL) st = launch_rays<0, 0>(wt, rv, table, a, n_rays, s);
-    else if (mode == NIQ_MODE_SLOPE_INTERVAL) st = launch_rays<1, 1>(wt, rv, table, a, n_rays, s);
-    else st = launch_rays<2, 0>(wt, rv, table, a, n_rays, s);
+    else if (mode == NIQ_MODE_AFFINE_FIXED) st = launch_rays<0, 1>(wt, rv, relu, table, a, n_rays, s);
+    else if (mode == NIQ_MODE_INTERVAL) st = launch_rays<0, 0>(wt, rv, relu, table, a, n_rays, s);
+    else if (mode == NIQ_MODE_SLOPE_INTERVAL) st = launch_rays<1, 1>(wt, rv, relu, table, a, n_rays, s);
+    else st = launch_rays<2, 0>(wt, rv, relu, table, a, n_rays, s);
   }
   long long hb_fix[2 * 1];
   std::vector<long long> hb_vec;

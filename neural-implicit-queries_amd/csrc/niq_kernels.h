@@ -597,7 +597,7 @@ struct RayArgs {
 // (segment centre, start, start + hit_eps) and the segment radius |half_vec|
 // kept in LDS per slot.
 template <int KIND, int V, int NB, int NW, int WMAX,
-          int OCC = LdsOcc<Planes<KIND, V, KIND == 2 ? 3 : 2>::R, NB, NW, WMAX>::value>
+          int OCC = LdsOcc<Planes<KIND, V, KIND == 2 ? 3 : 2>::R, NB, NW, WMAX>::value, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64, OCC)
 k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
   using PL = Planes<KIND, V, KIND == 2 ? 3 : 2>;
@@ -686,7 +686,7 @@ k_cast_rays(const DevModel* __restrict__ models, RayArgs a) {
             Rd[b] = sqrtf(__fmul_rn(nv, nv));
           }
         }
-        mlp_forward<PL, KIND, V, NB, NW, WMAX>(models[G.model0 + f], X, [&](const float* v, int b) {
+        mlp_forward<PL, KIND, V, NB, NW, WMAX, AS>(models[G.model0 + f], X, [&](const float* v, int b) {
           int code;
           if constexpr (KIND == 2) {
             code = sdf_code(v[0], Rd[b], G.lipschitz, 0.f);
