@@ -11,14 +11,14 @@ niq_status launch_points_dev_w(const niq_model* m, const float* x, const int* n_
 
 // A closest-point round as two launches: k_cp_forward (box and point tiles of
 // 16, grid sized for a full batch) and the one-workgroup update.
-template <int KIND, int V, int W>
+template <int KIND, int V, int W, int AS = ACTS_ALL>
 niq_status launch_cp_round_w(const niq_model* m, CpState* S, const CpArgs& a, hipStream_t s) {
   if constexpr (!Fits<Planes<KIND, V, 0>::R, 16, W>::value) {
     return fail(NIQ_ERR_UNSUPPORTED, "closest_point round: box planes above LDS");
   } else {
     constexpr int NW = TileCfgWide<W>::NW;
     const long long g_box = (a.B + 15) / 16, g_pts = (7LL * a.B + 15) / 16;
-    hipLaunchKernelGGL((k_cp_forward<KIND, V, NW, W>), dim3((unsigned)(g_box + g_pts)), dim3(NW * 64), 0, s,
+    hipLaunchKernelGGL((k_cp_forward<KIND, V, NW, W, AS>), dim3((unsigned)(g_box + g_pts)), dim3(NW * 64), 0, s,
                        m->dev, S, a, (int)g_box);
     hipLaunchKernelGGL(k_cp_update, dim3(1), dim3(1024), 0, s, S, a);
     HIPCHK(hipGetLastError());
@@ -28,6 +28,18 @@ niq_status launch_cp_round_w(const niq_model* m, CpState* S, const CpArgs& a, hi
 
 template <int KIND, int V>
 niq_status launch_cp_round_kv(const niq_model* m, CpState* S, const CpArgs& a, hipStream_t s) {
+  if constexpr (KIND == 0) {
+    // ReLU models: the round compiled for ReLU alone (bit-identical)
+    if (m->relu_only) {
+      switch (m->wt) {
+        case 64: return launch_cp_round_w<KIND, V, 64, ACTS_RELU>(m, S, a, s);
+        case 128: return launch_cp_round_w<KIND, V, 128, ACTS_RELU>(m, S, a, s);
+        case 256: return launch_cp_round_w<KIND, V, 256, ACTS_RELU>(m, S, a, s);
+        case 512: return launch_cp_round_w<KIND, V, 512, ACTS_RELU>(m, S, a, s);
+      }
+      return fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+  }
   switch (m->wt) {
     case 64: return launch_cp_round_w<KIND, V, 64>(m, S, a, s);
     case 128: return launch_cp_round_w<KIND, V, 128>(m, S, a, s);

@@ -279,7 +279,7 @@ __device__ __forceinline__ float cp_sample(const float* lo, const float* up, int
 // k_points_dev), so box and point tiles fill the chip together.  Every
 // (node, feature) and (point, feature) sums in the order of those kernels:
 // codes and values are theirs bit for bit.
-template <int KIND, int V, int NW, int WMAX>
+template <int KIND, int V, int NW, int WMAX, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64, (LdsOcc<Planes<KIND, V, 0>::R, 16, NW, WMAX>::value))
 k_cp_forward(const DevModel* __restrict__ Mp, CpState* S, CpArgs a, int g_box) {
   using PB = Planes<KIND, V, 0>;
@@ -311,7 +311,7 @@ k_cp_forward(const DevModel* __restrict__ Mp, CpState* S, CpArgs a, int g_box) {
     ca.upper = a.wup + 3LL * pop;
     ca.d = 3;
     fill_box_inputs<KIND, V, NB, NT, WMAX>(X, ca, i0, n);
-    mlp_forward<PB, KIND, V, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+    mlp_forward<PB, KIND, V, NB, NW, WMAX, AS>(*Mp, X, [&](const float* v, int b) {
       const long long i = i0 + b;
       if (i >= n) return;
       float lo, hi, scale;
@@ -332,7 +332,7 @@ k_cp_forward(const DevModel* __restrict__ Mp, CpState* S, CpArgs a, int g_box) {
       }
       X[xoff<NB, WMAX>(0, b, k)] = val;
     }
-    mlp_forward<PP, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+    mlp_forward<PP, 2, 0, NB, NW, WMAX, AS>(*Mp, X, [&](const float* v, int b) {
       const long long q = p0 + b;
       if (q < np) a.vals[q] = v[0];
     });
