@@ -185,10 +185,14 @@ niq_status niqi::classify_dispatch(const niq_model* m, const niq_eval_cfg* cfg, 
 }
 
 namespace {
-template <int W>
+// AS: the activation set of the kernel (ReLU models take the ReLU-only one).
+template <int W, int AS = ACTS_ALL>
 niq_status launch_points_w(const niq_model* m, const float* x, int d, long long n, float* out, hipStream_t s) {
+  if constexpr (AS == ACTS_ALL) {
+    if (m->relu_only) return launch_points_w<W, ACTS_RELU>(m, x, d, n, out, s);
+  }
   constexpr int NB = TileCfg<W>::NBP, NW = TileCfg<W>::NW;
-  auto k = k_points<NB, NW, W>;
+  auto k = k_points<NB, NW, W, AS>;
   const long long tiles = (n + NB - 1) / NB;
   if (tiles == 0) return NIQ_OK;
   const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));

@@ -5,7 +5,7 @@
 // ---------------------------------------------------------- closest point
 namespace {
 
-template <int W>
+template <int W, int AS = ACTS_ALL>
 niq_status launch_points_dev_w(const niq_model* m, const float* x, const int* n_dev, int mult, long long max_n,
                                float* out, hipStream_t s);
 
@@ -350,11 +350,15 @@ extern "C" niq_status niq_mc_extract(const niq_model* m, const float* leaf_lower
 }
 
 namespace {
-template <int W>
+// AS: the activation set of the kernel (ReLU models take the ReLU-only one).
+template <int W, int AS>
 niq_status launch_points_dev_w(const niq_model* m, const float* x, const int* n_dev, int mult, long long max_n,
                                float* out, hipStream_t s) {
+  if constexpr (AS == ACTS_ALL) {
+    if (m->relu_only) return launch_points_dev_w<W, ACTS_RELU>(m, x, n_dev, mult, max_n, out, s);
+  }
   constexpr int NB = TileCfg<W>::NBP, NW = TileCfg<W>::NW;
-  auto k = k_points_dev<NB, NW, W>;
+  auto k = k_points_dev<NB, NW, W, AS>;
   const long long tiles = (max_n + NB - 1) / NB;
   if (tiles == 0) return NIQ_OK;
   const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));

@@ -13,11 +13,15 @@ k_hit_positions(const float* __restrict__ roots, const float* __restrict__ dirs,
 
 unsigned elem_grid(long long n) { return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 16384)); }
 
-template <int W>
+// AS: the activation set of the kernel (ReLU models take the ReLU-only one).
+template <int W, int AS = ACTS_ALL>
 niq_status launch_normal_eval(const niq_model* m, const int* list, const int* count, long long max_n,
                               const float* hit_pos, float eps, float* vals, hipStream_t s) {
+  if constexpr (AS == ACTS_ALL) {
+    if (m->relu_only) return launch_normal_eval<W, ACTS_RELU>(m, list, count, max_n, hit_pos, eps, vals, s);
+  }
   constexpr int NB = TileCfg<W>::NBP, NW = TileCfg<W>::NW;
-  auto k = k_normal_eval<NB, NW, W>;
+  auto k = k_normal_eval<NB, NW, W, AS>;
   const long long tiles = (4 * max_n + NB - 1) / NB;
   if (tiles == 0) return NIQ_OK;
   const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));

@@ -24,11 +24,16 @@ bool carve(void* ws, size_t bytes, long long n_per_round, SampleWs& w) {
   return off <= bytes;
 }
 
-template <int W, bool MASS>
+// AS: the activation set of the kernel (ReLU models take the ReLU-only one).
+template <int W, bool MASS, int AS = ACTS_ALL>
 niq_status launch_sample_eval(const niq_model* m, const SampleSrc& S, long long n, float width, unsigned char* flags,
                               Sum4* partial, int max_blocks, int* blocks_used, hipStream_t s) {
+  if constexpr (AS == ACTS_ALL) {
+    if (m->relu_only)
+      return launch_sample_eval<W, MASS, ACTS_RELU>(m, S, n, width, flags, partial, max_blocks, blocks_used, s);
+  }
   constexpr int NB = TileCfg<W>::NBP, NW = TileCfg<W>::NW;
-  auto k = k_sample_eval<NB, NW, W, MASS>;
+  auto k = k_sample_eval<NB, NW, W, MASS, AS>;
   const long long tiles = (n + NB - 1) / NB;
   unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
   if (max_blocks > 0) g = std::min<unsigned>(g, (unsigned)max_blocks);

@@ -304,7 +304,7 @@ k_recheck_fp64(const DevModel* __restrict__ Mp, ClassifyArgs a) {
 }
 
 // ------------------------------------------------------------ point values
-template <int NB, int NW, int WMAX>
+template <int NB, int NW, int WMAX, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64)
 k_points(const DevModel* __restrict__ Mp, const float* __restrict__ x, int d, long long n,
          float* __restrict__ out) {
@@ -319,7 +319,7 @@ k_points(const DevModel* __restrict__ Mp, const float* __restrict__ x, int d, lo
       const long long i = i0 + b;
       X[xoff<NB, WMAX>(0, b, k)] = (i < n && k < d) ? x[i * d + k] : 0.f;
     }
-    mlp_forward<PL, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+    mlp_forward<PL, 2, 0, NB, NW, WMAX, AS>(*Mp, X, [&](const float* v, int b) {
       const long long i = i0 + b;
       if (i < n) out[i] = v[0];
     });
@@ -328,7 +328,7 @@ k_points(const DevModel* __restrict__ Mp, const float* __restrict__ x, int d, lo
 
 // Point values of n = (*n_dev) * mult points (3D), or of n_host points when
 // n_dev is null.
-template <int NB, int NW, int WMAX>
+template <int NB, int NW, int WMAX, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64)
 k_points_dev(const DevModel* __restrict__ Mp, const float* __restrict__ x, const int* __restrict__ n_dev, int mult,
              long long n_host, float* __restrict__ out) {
@@ -344,7 +344,7 @@ k_points_dev(const DevModel* __restrict__ Mp, const float* __restrict__ x, const
       const long long i = i0 + b;
       X[xoff<NB, WMAX>(0, b, k)] = (i < n && k < 3) ? x[i * 3 + k] : 0.f;
     }
-    mlp_forward<PL, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+    mlp_forward<PL, 2, 0, NB, NW, WMAX, AS>(*Mp, X, [&](const float* v, int b) {
       const long long i = i0 + b;
       if (i < n) out[i] = v[0];
     });

@@ -56,7 +56,7 @@ k_hit_lists(const int* __restrict__ hit_ids, long long n, int n_models, long lon
 }
 
 // Values of the 4 tetrahedron samples around each listed pixel: vals[4 s + q].
-template <int NB, int NW, int WMAX>
+template <int NB, int NW, int WMAX, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64)
 k_normal_eval(const DevModel* __restrict__ Mp, const int* __restrict__ list, const int* __restrict__ count,
               const float* __restrict__ hit_pos, float eps, float* __restrict__ vals) {
@@ -77,7 +77,7 @@ k_normal_eval(const DevModel* __restrict__ Mp, const int* __restrict__ list, con
       }
       X[xoff<NB, WMAX>(0, b, k)] = val;
     }
-    mlp_forward<PL, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+    mlp_forward<PL, 2, 0, NB, NW, WMAX, AS>(*Mp, X, [&](const float* v, int b) {
       const long long j = i0 + b;
       if (j < n) vals[j] = v[0];
     });

@@ -64,7 +64,7 @@ __device__ __forceinline__ void sample_candidate(const SampleSrc& S, unsigned lo
 // n candidates S.j0 .. S.j0 + n - 1 through the fused point pass.  MASS:
 // per-workgroup count and position sums of the candidates with f < 0 into
 // partial[blockIdx.x]; otherwise flags[i] = |f| < width.
-template <int NB, int NW, int WMAX, bool MASS>
+template <int NB, int NW, int WMAX, bool MASS, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64)
 k_sample_eval(const DevModel* __restrict__ Mp, SampleSrc S, long long n, float width,
               unsigned char* __restrict__ flags, Sum4* __restrict__ partial) {
@@ -90,7 +90,7 @@ k_sample_eval(const DevModel* __restrict__ Mp, SampleSrc S, long long n, float w
         if constexpr (MASS) P[3 * b + k] = p[k];
       }
     }
-    mlp_forward<PL, 2, 0, NB, NW, WMAX>(*Mp, X, [&](const float* v, int b) {
+    mlp_forward<PL, 2, 0, NB, NW, WMAX, AS>(*Mp, X, [&](const float* v, int b) {
       const long long i = i0 + b;
       if (i >= n) return;
       if constexpr (MASS) {
