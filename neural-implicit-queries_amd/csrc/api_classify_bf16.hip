@@ -16,13 +16,13 @@ bool lw_eligible(const niq_model* m) {
   return true;
 }
 
-template <int V, int W, int WS>
+template <int V, int W, int WS, int AS = ACTS_ALL>
 niq_status launch_bf16_w(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
   constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
   if constexpr ((size_t)PlanesBf16<V>::NPL * NB * W * 2 > (size_t)kLdsLimit) {
     return fail(NIQ_ERR_UNSUPPORTED, "precision-mode range form does not fit in LDS at this width");
   } else {
-    auto k = k_classify_bf16<V, NB, NW, W, WS>;
+    auto k = k_classify_bf16<V, NB, NW, W, WS, AS>;
     const long long tiles = (max_n + NB - 1) / NB;
     if (tiles == 0) return NIQ_OK;
     const unsigned g = tile_grid(tiles, persistent_blocks(k, NW * 64));
@@ -34,6 +34,18 @@ niq_status launch_bf16_w(const niq_model* m, const ClassifyArgs& a, long long ma
 
 template <int V, int WS>
 niq_status launch_bf16_ws(const niq_model* m, const ClassifyArgs& a, long long max_n, hipStream_t s) {
+  if constexpr (WS == WSRC_FP32) {
+    // ReLU models: the kernel compiled for ReLU alone (bit-identical)
+    if (m->relu_only) {
+      switch (m->wt16) {
+        case 64: return launch_bf16_w<V, 64, WS, ACTS_RELU>(m, a, max_n, s);
+        case 128: return launch_bf16_w<V, 128, WS, ACTS_RELU>(m, a, max_n, s);
+        case 256: return launch_bf16_w<V, 256, WS, ACTS_RELU>(m, a, max_n, s);
+        case 512: return launch_bf16_w<V, 512, WS, ACTS_RELU>(m, a, max_n, s);
+      }
+      return fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+  }
   switch (m->wt16) {
     case 64: return launch_bf16_w<V, 64, WS>(m, a, max_n, s);
     case 128: return launch_bf16_w<V, 128, WS>(m, a, max_n, s);

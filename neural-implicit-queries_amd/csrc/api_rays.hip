@@ -63,13 +63,13 @@ niq_status launch_rays(int wt, int variant, bool relu_only, const DevModel* mode
   return fail(NIQ_ERR_INVALID, "ray_variant must be 0, 1 or 2");
 }
 
-template <int V, int W>
+template <int V, int W, int AS = ACTS_ALL>
 niq_status launch_rays_bf16_w(const DevModelBf16* models, const RayArgs& a, long long n_rays, hipStream_t s) {
   constexpr int NB = TileCfg<W>::NB, NW = TileCfg<W>::NW;
   if constexpr (!RayPlanesBf16<V, NB, W>::fits) {
     return fail(NIQ_ERR_UNSUPPORTED, "precision-mode ray form does not fit in LDS at this width");
   } else {
-    auto k = k_cast_rays_bf16<V, NB, NW, W>;
+    auto k = k_cast_rays_bf16<V, NB, NW, W, AS>;
     const long long tiles = (n_rays + NB - 1) / NB;
     const unsigned g = grid_for(tiles, persistent_blocks(k, NW * 64));
     hipLaunchKernelGGL(k, dim3(g), dim3(NW * 64), 0, s, models, a);
@@ -79,7 +79,17 @@ niq_status launch_rays_bf16_w(const DevModelBf16* models, const RayArgs& a, long
 }
 
 template <int V>
-niq_status launch_rays_bf16(int wt, const DevModelBf16* models, const RayArgs& a, long long n_rays, hipStream_t s) {
+niq_status launch_rays_bf16(int wt, bool relu_only, const DevModelBf16* models, const RayArgs& a, long long n_rays,
+                            hipStream_t s) {
+  if (relu_only) {
+    switch (wt) {
+      case 64: return launch_rays_bf16_w<V, 64, ACTS_RELU>(models, a, n_rays, s);
+      case 128: return launch_rays_bf16_w<V, 128, ACTS_RELU>(models, a, n_rays, s);
+      case 256: return launch_rays_bf16_w<V, 256, ACTS_RELU>(models, a, n_rays, s);
+      case 512: return launch_rays_bf16_w<V, 512, ACTS_RELU>(models, a, n_rays, s);
+    }
+    return fail(NIQ_ERR_UNSUPPORTED, "width");
+  }
   switch (wt) {
     case 64: return launch_rays_bf16_w<V, 64>(models, a, n_rays, s);
     case 128: return launch_rays_bf16_w<V, 128>(models, a, n_rays, s);
@@ -264,8 +274,8 @@ niq_status cast_run_groups(CastGroupReq* reqs, int ng, const niq_eval_cfg* cfg, 
     RayArgs a{};
     a.groups = gtab;
     a.n_groups = ng;
-    if (bf16) st = mode == NIQ_MODE_AFFINE_FIXED ? launch_rays_bf16<1>(wt16, table16, a, n_rays, s)
-                                                 : launch_rays_bf16<0>(wt16, table16, a, n_rays, s);
+    if (bf16) st = mode == NIQ_MODE_AFFINE_FIXED ? launch_rays_bf16<1>(wt16, relu, table16, a, n_rays, s)
+                                                 : launch_rays_bf16<0>(wt16, relu, table16, a, n_rays, s);
     else if (mode == NIQ_MODE_AFFINE_FIXED) st = launch_rays<0, 1>(wt, rv, relu, table, a, n_rays, s);
     else if (mode == NIQ_MODE_INTERVAL) st = launch_rays<0, 0>(wt, rv, relu, table, a, n_rays, s);
     else if (mode == NIQ_MODE_SLOPE_INTERVAL) st = launch_rays<1, 1>(wt, rv, relu, table, a, n_rays, s);

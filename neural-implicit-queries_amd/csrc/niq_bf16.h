@@ -284,7 +284,7 @@ __device__ __forceinline__ void split_out4(float (&v)[4][NV], float cb_next, flo
 // NP plain point rows (for the ray caster's hit test) ride after the range
 // form, each split NPS ways (planes PB::NPL + r * NPS + s); they get the
 // plain activation and reach `fin` in v[V + 2 + r].
-template <int V, int NB, int NW, int WMAX, int WS, int NP = 0, int NPS = 3, class Fin>
+template <int V, int NB, int NW, int WMAX, int WS, int NP = 0, int NPS = 3, int AS = ACTS_ALL, class Fin>
 __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned short* __restrict__ Xs, Fin&& fin) {
   using PB = PlanesBf16<V>;
   constexpr int NS = V + 1;
@@ -450,11 +450,20 @@ __device__ __forceinline__ void mlp_forward_bf16(const DevModelBf16& M, unsigned
           if (ft == 0 && lg == 0) fin(v[0], b);
           continue;
         }
+        auto hidden = [&](int act) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          affine_rule_rigorous<V>(Ly.act, v[j]);
+          for (int j = 0; j < 4; ++j) {
+            affine_rule_rigorous<V>(act, v[j]);
 #pragma unroll
-          for (int r = 0; r < NP; ++r) v[j][V + 2 + r] = act_point(Ly.act, v[j][V + 2 + r]);
+            for (int r = 0; r < NP; ++r) v[j][V + 2 + r] = act_point(act, v[j][V + 2 + r]);
+          }
+        };
+        if constexpr (AS == ACTS_RELU) {
+          // the rules with the activation a constant (ReLU, or the identity)
+          if (Ly.act == ACT_RELU) hidden(ACT_RELU);
+          else hidden(ACT_NONE);
+        } else {
+          hidden(Ly.act);
         }
         // next layer's planes; its contraction bound rides in the e plane
         OutPlanes<V> o;
@@ -500,7 +509,7 @@ __device__ __forceinline__ void final_bounds_fp64(const float* v, double& lo, do
   scale = ::fabs((double)v[0]) + rad;
 }
 
-template <int V, int NB, int NW, int WMAX, int WS>
+template <int V, int NB, int NW, int WMAX, int WS, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64, (BfOcc<V, NB, NW, WMAX>::value))
 k_classify_bf16(const DevModelBf16* __restrict__ Mp, ClassifyArgs a) {
   using PB = PlanesBf16<V>;
@@ -539,7 +548,7 @@ k_classify_bf16(const DevModelBf16* __restrict__ Mp, ClassifyArgs a) {
       }
       store_split<V, NB, WMAX>(Xs, b, k, s, err, cb0, ca0);
     }
-    mlp_forward_bf16<V, NB, NW, WMAX, WS>(*Mp, Xs, [&](const float* v, int b) {
+    mlp_forward_bf16<V, NB, NW, WMAX, WS, 0, 3, AS>(*Mp, Xs, [&](const float* v, int b) {
       const long long i = i0 + b;
       if (i >= n) return;
       double lo, hi, scale;

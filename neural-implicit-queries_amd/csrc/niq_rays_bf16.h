@@ -16,7 +16,7 @@ struct RayPlanesBf16 {
   static constexpr bool fits = (size_t)NPT * NB * WMAX * 2 <= (size_t)kLdsLimit;
 };
 
-template <int V, int NB, int NW, int WMAX>
+template <int V, int NB, int NW, int WMAX, int AS = ACTS_ALL>
 __global__ void __launch_bounds__(NW * 64)
 k_cast_rays_bf16(const DevModelBf16* __restrict__ models, RayArgs a) {
   using RP = RayPlanesBf16<V, NB, WMAX>;
@@ -107,7 +107,7 @@ k_cast_rays_bf16(const DevModelBf16* __restrict__ models, RayArgs a) {
             }
           }
         }
-        mlp_forward_bf16<V, NB, NW, WMAX, WSRC_FP32, 2, PS>(M, Xs, [&](const float* v, int b) {
+        mlp_forward_bf16<V, NB, NW, WMAX, WSRC_FP32, 2, PS, AS>(M, Xs, [&](const float* v, int b) {
           double lo, hi, scale;
           final_bounds_fp64<V>(v, lo, hi, scale);
           Xf[4 * b + 0] = __int_as_float(sign_code<double>(lo, hi, 0.0));
