@@ -1195,3 +1195,55 @@ def test_bf16_weight_sources_agree(name):
     for v in (1, 2):
         b = out[v][1]
         assert np.all(vals >= b[:, :1]) and np.all(vals <= b[:, 1:])
+
+
+def test_relu_only_kernels_with_identity_layers():
+    """A ReLU model whose middle hidden layer has no activation (the identity)
+    runs the kernels compiled for ReLU alone: classification through the
+    default tile equals a tile variant that keeps the generic kernel bit for
+    bit and the oracle within its bars; casts equal the generic ray variant;
+    point values equal the oracle's."""
+    note("[relu-only kernels] 3->48->40->32->1, relu / identity / relu")
+    rng = np.random.default_rng(11)
+    dims, acts = [3, 48, 40, 32, 1], ["relu", None, "relu"]
+    ops = []
+    for i in range(4):
+        A = rng.normal(0.0, np.sqrt(2.0 / dims[i]), size=(dims[i], dims[i + 1])).astype(np.float32)
+        b = rng.normal(0.0, 0.1, size=(dims[i + 1],)).astype(np.float32)
+        ops.append(nmlp.dense(dims[i], dims[i + 1], A=A, b=b))
+        if i < 3 and acts[i]:
+            ops.append(nmlp.relu())
+    ops.append(nmlp.squeeze_last())
+    p = nmlp.build_spec(ops)
+    func, _ = implicit_mlp_utils.implicit_from_params(p, "affine_fixed")
+    lo, up = boxes(2000, seed=5, scale=0.5)
+    tlo, tup = torch.as_tensor(lo, device="cuda"), torch.as_tensor(up, device="cuda")
+    out = {}
+    for v in (0, 2):   # 0: the ReLU-only kernel; 2: the twice-the-boxes tile, generic kernel
+        func.cfg.variant = v
+        c, b = func.classify_box(p, tlo, tup, return_bounds=True)
+        out[v] = (c.cpu().numpy(), b.cpu().numpy())
+    func.cfg.variant = 0
+    assert np.array_equal(out[0][0], out[2][0]) and np.array_equal(out[0][1], out[2][1])
+    of32 = ofuncs.OracleFunc(p, "affine_fixed", np.float32)
+    of64 = ofuncs.OracleFunc(p, "affine_fixed", np.float64)
+    r_lo, r_hi = of32.bounds_box(lo, up)
+    d_lo, d_hi = of64.bounds_box(lo, up)
+    bounds = out[0][1]
+    noise_lo, noise_hi = np.abs(r_lo - d_lo), np.abs(r_hi - d_hi)
+    assert np.all(np.abs(bounds[:, 0] - r_lo) <= 1e-5 * np.max(np.abs(r_lo)) + 2 * noise_lo)
+    assert np.all(np.abs(bounds[:, 1] - r_hi) <= 1e-5 * np.max(np.abs(r_hi)) + 2 * noise_hi)
+    x = np.random.default_rng(0).uniform(-1, 1, size=(3000, 3)).astype(np.float32)
+    got = func(p, torch.as_tensor(x, device="cuda")).cpu().numpy()
+    assert np.all(rel_close(got, omlp.eval_points(p, x, np.float32)))
+    roots, dirs = camera(48)
+    opts = queries.get_default_cast_opts()
+    opts["max_dist"] = 6.0
+    casts = {}
+    for v in (0, 1):   # 0: the ReLU-only kernel; 1: the same tile, generic kernel
+        func.cfg.ray_variant = v
+        casts[v] = [x.cpu().numpy() if hasattr(x, "cpu") else x
+                    for x in queries.cast_rays((func,), (p,), roots, dirs, opts)]
+    func.cfg.ray_variant = 0
+    assert all(np.array_equal(a, b) for a, b in zip(casts[0][:3], casts[1][:3])) and casts[0][3] == casts[1][3]
+    note(f"  codes determinate {(out[0][0] != 0).sum()}, hits {(casts[0][1] > 0).sum()}")
