@@ -29,13 +29,23 @@ niq_status launch_cp_round_w(const niq_model* m, CpState* S, const CpArgs& a, hi
 template <int KIND, int V>
 niq_status launch_cp_round_kv(const niq_model* m, CpState* S, const CpArgs& a, hipStream_t s) {
   if constexpr (KIND == 0) {
-    // ReLU models: the round compiled for ReLU alone (bit-identical)
+    // ReLU or TanH models: the round compiled for that activation alone
+    // (bit-identical; fewer registers, so more workgroups per CU)
     if (m->relu_only) {
       switch (m->wt) {
         case 64: return launch_cp_round_w<KIND, V, 64, ACTS_RELU>(m, S, a, s);
         case 128: return launch_cp_round_w<KIND, V, 128, ACTS_RELU>(m, S, a, s);
         case 256: return launch_cp_round_w<KIND, V, 256, ACTS_RELU>(m, S, a, s);
         case 512: return launch_cp_round_w<KIND, V, 512, ACTS_RELU>(m, S, a, s);
+      }
+      return fail(NIQ_ERR_UNSUPPORTED, "width");
+    }
+    if (m->tanh_only) {
+      switch (m->wt) {
+        case 64: return launch_cp_round_w<KIND, V, 64, ACTS_TANH>(m, S, a, s);
+        case 128: return launch_cp_round_w<KIND, V, 128, ACTS_TANH>(m, S, a, s);
+        case 256: return launch_cp_round_w<KIND, V, 256, ACTS_TANH>(m, S, a, s);
+        case 512: return launch_cp_round_w<KIND, V, 512, ACTS_TANH>(m, S, a, s);
       }
       return fail(NIQ_ERR_UNSUPPORTED, "width");
     }

@@ -76,8 +76,11 @@ niq_status niq_model_create(int n_layers, const int* dims, const int* acts, cons
   m->acts.assign(acts, acts + n_layers);
   // hidden layers ReLU or the identity: the ReLU-only classify kernels apply
   m->relu_only = true;
-  for (int i = 0; i + 1 < n_layers; ++i)
+  m->tanh_only = true;
+  for (int i = 0; i + 1 < n_layers; ++i) {
     if (acts[i] != NIQ_ACT_RELU && acts[i] != NIQ_ACT_NONE) m->relu_only = false;
+    if (acts[i] != NIQ_ACT_TANH && acts[i] != NIQ_ACT_NONE) m->tanh_only = false;
+  }
   hipError_t e = hipMalloc(&m->weights, total);
   if (e == hipSuccess) e = hipMemcpy(m->weights, host.data(), total, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&m->dev), sizeof(DevModel));

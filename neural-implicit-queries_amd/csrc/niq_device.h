@@ -362,16 +362,17 @@ __device__ __forceinline__ void final_bounds(const T* v, T& lo, T& hi, T& scale)
   }
 }
 
-// Activation sets a kernel is compiled for: every activation, or ReLU alone
-// (models whose hidden layers are all ReLU; the rules of the others and their
-// constants then take no registers).
-enum { ACTS_ALL = 0, ACTS_RELU = 1 };
+// Activation sets a kernel is compiled for: every activation, or ReLU or TanH
+// alone (models whose hidden layers all use that one; the rules of the others
+// and their constants then take no registers).
+enum { ACTS_ALL = 0, ACTS_RELU = 1, ACTS_TANH = 2 };
 
 template <class PL, int KIND, int V, int AS = ACTS_ALL, class T>
 __device__ __forceinline__ void apply_hidden(int act, T* v) {
-  if constexpr (AS == ACTS_RELU) {
+  if constexpr (AS == ACTS_RELU || AS == ACTS_TANH) {
     // the same rule with the activation a constant (identity otherwise)
-    if (act == ACT_RELU) apply_hidden<PL, KIND, V, ACTS_ALL>(ACT_RELU, v);
+    constexpr int A = AS == ACTS_RELU ? ACT_RELU : ACT_TANH;
+    if (act == A) apply_hidden<PL, KIND, V, ACTS_ALL>(A, v);
     return;
   }
   if constexpr (KIND == 0) affine_fixed_rule<V>(act, v);

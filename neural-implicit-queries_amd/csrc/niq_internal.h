@@ -127,6 +127,7 @@ struct niq_model {
   int wt16 = 0;                // template width of the precision-mode copy
   bool bf16_ok = true;         // no FP32 subnormal weights (precision mode)
   bool relu_only = false;      // every hidden activation is ReLU
+  bool tanh_only = false;      // every hidden activation is TanH
   DevModelBf16 host16;         // precision-mode copy (bf16 hi/lo/|W| weights)
   DevModelBf16* dev16 = nullptr;
   void* weights16 = nullptr;
