@@ -145,7 +145,7 @@ extern "C" niq_status niq_closest_point(const niq_model* m, const niq_eval_cfg* 
   }
   a.B = B; a.cap = cap;
   a.bq = (int*)pbq; a.blo = (float*)pblo; a.bup = (float*)pbup; a.pts = (float*)ppts;
-  a.codes = (int*)pcodes; a.vals = (float*)pvals; a.q = query_points;
+  a.codes = (int*)pcodes; a.vals = (float*)pvals; a.q = query_points; a.Q = Q;
   a.qmin = out_dist; a.qloc = out_loc; a.win = (int*)pwin;
   a.idist = (float*)pidist; a.ineeds = (int*)pineeds;
   // eps / sqrt(d) in float32 (src/kd_tree.py:688)

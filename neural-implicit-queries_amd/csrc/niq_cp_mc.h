@@ -36,6 +36,7 @@ struct CpArgs {
   int* codes;                   // (B,)
   float* vals;                  // (7B,)
   const float* q;               // query points (Q, 3)
+  long long Q;                  // query count
   float* qmin;                  // (Q,) running minimum distance
   float* qloc;                  // (Q, 3)
   int* win;                     // (Q,) scratch, -1 between rounds
@@ -111,7 +112,7 @@ __device__ __forceinline__ void cp_push_children(const CpArgs& a, int pop, int r
 // (its centre distance plus the node diagonal when the samples change sign)
 // and whether it is subdivided (src/kd_tree.py:692-716).
 __device__ __forceinline__ void cp_item(const CpArgs& a, int j, int qid, const float* lo, const float* up,
-                                        float& idist, int& needs) {
+                                        float qmin_pre, float& idist, int& needs) {
   float c[3], e[3];
   for (int k = 0; k < 3; ++k) { c[k] = 0.5f * (lo[k] + up[k]); e[k] = up[k] - lo[k]; }
   const float width = fmaxf(fmaxf(e[0], e[1]), e[2]);
@@ -127,17 +128,33 @@ __device__ __forceinline__ void cp_item(const CpArgs& a, int j, int qid, const f
     all_pos = all_pos && (v > 0.f);
   }
   idist = !(all_neg || all_pos) ? dc + diag : INFINITY;
-  needs = (!outside && !is_small && dc < load_fresh(&a.qmin[qid])) ? 1 : 0;
+  needs = (!outside && !is_small && dc < qmin_pre) ? 1 : 0;
 }
+
+// Queries whose running minimum and winning item the register-resident update
+// keeps in LDS for the round (one coalesced load and store of the minima
+// instead of scattered L2 atomics and reads per item).
+constexpr int kCpLdsQ = 4096;
 
 // The update for n <= 2 NT items (the default batch of 2048 on 1024
 // threads): item j = threadIdx + k NT stays in thread registers through all
 // phases, and the stable compaction scans slice k = 0, 1 in turn, so ranks
-// follow the item order as in cp_update_body.
+// follow the item order as in cp_update_body.  With at most kCpLdsQ queries
+// the minima and winners live in LDS for the round: the same atomics (integer
+// min over non-negative float bits, max item index) in the same phases, so
+// the results are those of the global form.
 template <int NT>
 __device__ __forceinline__ void cp_update_regs(CpState* S, const CpArgs& a, int n, int pop) {
   using Scan = rocprim::block_scan<int, NT>;
   __shared__ typename Scan::storage_type scan_storage;
+  __shared__ float s_qmin[kCpLdsQ];
+  __shared__ int s_win[kCpLdsQ];
+  const bool lds = a.Q <= kCpLdsQ;   // uniform
+  const int nq = (int)a.Q;
+  if (lds) {
+    for (int i = threadIdx.x; i < nq; i += NT) { s_qmin[i] = a.qmin[i]; s_win[i] = -1; }
+    __syncthreads();
+  }
   constexpr int KI = 2;
   int qid[KI], needs[KI];
   float lo[KI][3], up[KI][3], idist[KI];
@@ -148,31 +165,39 @@ __device__ __forceinline__ void cp_update_regs(CpState* S, const CpArgs& a, int 
     if (j < n) {
       qid[k] = a.bq[j];
       for (int d = 0; d < 3; ++d) { lo[k][d] = a.blo[3 * j + d]; up[k][d] = a.bup[3 * j + d]; }
-      cp_item(a, j, qid[k], lo[k], up[k], idist[k], needs[k]);
+      const float pre = lds ? s_qmin[qid[k]] : load_fresh(&a.qmin[qid[k]]);
+      cp_item(a, j, qid[k], lo[k], up[k], pre, idist[k], needs[k]);
     }
   }
   __syncthreads();   // every pre-round minimum is read before the scatter
 #pragma unroll
   for (int k = 0; k < KI; ++k)
-    if (threadIdx.x + k * NT < n && idist[k] < INFINITY)
-      atomicMin(reinterpret_cast<int*>(&a.qmin[qid[k]]), __float_as_int(idist[k]));
+    if (threadIdx.x + k * NT < n && idist[k] < INFINITY) {
+      int* m = lds ? reinterpret_cast<int*>(&s_qmin[qid[k]]) : reinterpret_cast<int*>(&a.qmin[qid[k]]);
+      atomicMin(m, __float_as_int(idist[k]));
+    }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KI; ++k) {
     const int j = threadIdx.x + k * NT;
-    if (j < n && idist[k] == load_fresh(&a.qmin[qid[k]])) atomicMax(&a.win[qid[k]], j);
+    if (j < n && idist[k] == (lds ? s_qmin[qid[k]] : load_fresh(&a.qmin[qid[k]])))
+      atomicMax(lds ? &s_win[qid[k]] : &a.win[qid[k]], j);
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KI; ++k) {
     const int j = threadIdx.x + k * NT;
-    if (j < n && load_fresh(&a.win[qid[k]]) == j)
+    if (j < n && (lds ? s_win[qid[k]] : load_fresh(&a.win[qid[k]])) == j)
       for (int d = 0; d < 3; ++d) a.qloc[3 * qid[k] + d] = 0.5f * (lo[k][d] + up[k][d]);
   }
   __syncthreads();
+  if (lds) {
+    for (int i = threadIdx.x; i < nq; i += NT) a.qmin[i] = s_qmin[i];
+  } else {
 #pragma unroll
-  for (int k = 0; k < KI; ++k)
-    if (threadIdx.x + k * NT < n) a.win[qid[k]] = -1;
+    for (int k = 0; k < KI; ++k)
+      if (threadIdx.x + k * NT < n) a.win[qid[k]] = -1;
+  }
   int base = 0;
 #pragma unroll
   for (int k = 0; k < KI; ++k) {
@@ -207,7 +232,7 @@ __device__ __forceinline__ void cp_update_body(CpState* S, const CpArgs& a, int 
     for (int k = 0; k < 3; ++k) { lo[k] = a.blo[3 * j + k]; up[k] = a.bup[3 * j + k]; }
     float idist;
     int needs;
-    cp_item(a, j, a.bq[j], lo, up, idist, needs);
+    cp_item(a, j, a.bq[j], lo, up, load_fresh(&a.qmin[a.bq[j]]), idist, needs);
     a.idist[j] = idist;
     a.ineeds[j] = needs;
   }
